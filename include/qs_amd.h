@@ -1,0 +1,191 @@
+/*
+ * qs_amd.h -- C ABI of the MI355X (gfx950) block-scatter tensor-contraction
+ * library.  Plain C types only: integers, doubles, pointers and sizes.
+ *
+ * The host planner (QuasiStrided.jl's `plan_contract`, or the Python mirror in
+ * this package) resolves labels into M/N/K axis groups exactly as
+ * src/planning/plan.jl:197-262 does, then hands the groups, the operand
+ * storages and the folded conjugation flags to `qs_plan_create`.  The library
+ * owns everything device-side: offset tables, tile choice, scratch.
+ *
+ * Error convention (replaces Julia exceptions, src/layout/axis_group.jl:13-41,
+ * src/planning/labels.jl:28-102, src/microkernels/interface.jl:164-206): every
+ * entry point returns a qs_status; a non-zero status leaves every output
+ * untouched ("before any write") and `qs_last_error()` holds a message.
+ * Nothing throws across the ABI.
+ *
+ * Threading: a plan is used by one thread at a time (src/planning/plan.jl:12-13);
+ * library globals (device context, autotune cache) are thread-safe.  Every
+ * execute takes an explicit HIP stream (hipStream_t passed as void*; NULL is
+ * the default stream).
+ */
+#ifndef QS_AMD_H
+#define QS_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: one per reference exception class ------------------- */
+typedef enum qs_status {
+  QS_OK = 0,
+  QS_ERR_ARGUMENT = 1,    /* ArgumentError    */
+  QS_ERR_DIMENSION = 2,   /* DimensionMismatch */
+  QS_ERR_BOUNDS = 3,      /* BoundsError      */
+  QS_ERR_OVERFLOW = 4,    /* OverflowError    */
+  QS_ERR_DEVICE = 5,      /* a HIP runtime error */
+  QS_ERR_MEMORY = 6,      /* device allocation failed or budget exceeded */
+  QS_ERR_UNSUPPORTED = 7  /* valid input this build does not implement */
+} qs_status;
+
+/* ---- element types (src/planning/plan.jl:264 SUPPORTED_ELTYPES) -------- */
+typedef enum qs_dtype {
+  QS_F32 = 0,   /* Float32    */
+  QS_F64 = 1,   /* Float64    */
+  QS_C32 = 2,   /* ComplexF32 (interleaved re, im) */
+  QS_C64 = 3    /* ComplexF64 (interleaved re, im) */
+} qs_dtype;
+
+#define QS_MAX_DIMS 16
+
+/* One grouped axis with two stride maps: AxisGroup{D,2}
+ * (src/layout/axis_group.jl:5-41).  `ndim` dims, enumerated by one zero-based
+ * logical coordinate, first dim fastest; strides in elements, zero-based, no
+ * base.  M groups carry maps (A, C), N groups (B, C), K groups (A, B) -- after
+ * the planner's M/N swap, "A" is whichever operand feeds M. */
+typedef struct qs_group {
+  int32_t ndim;
+  int32_t _pad;
+  int64_t lengths[QS_MAX_DIMS];
+  int64_t strides[2][QS_MAX_DIMS];
+} qs_group;
+
+/* Where an operand's storage lives. */
+typedef enum qs_memory {
+  QS_MEM_DEVICE = 0,  /* device pointer: the library reads/writes it in place   */
+  QS_MEM_HOST = 1     /* host pointer: the library stages the touched span H2D,
+                         and C's span back D2H, on each execute */
+} qs_memory;
+
+/* Precision modes. */
+typedef enum qs_precision {
+  QS_PLAIN = 0,       /* native FP64/FP32 MFMA in the compute type            */
+  QS_SPLIT_INT8 = 1   /* FP64/FP32 from INT8 slices of scaled operands
+                         (Ozaki scheme) on the INT8 MFMA units               */
+} qs_precision;
+
+/* Everything `ContractPlan` (src/planning/plan.jl:16-45) carries that the
+ * device needs.  Operands: storage pointer, zero-based element offset of the
+ * view (StridedViews `offset`), storage length in elements (for the exact span
+ * check, src/layout/tiles.jl:123-134). */
+typedef struct qs_contract_desc {
+  int32_t a_type, b_type, c_type;   /* qs_dtype of each storage              */
+  int32_t compute_type;             /* qs_dtype T (plan.jl:278-285)          */
+  int32_t conj_a, conj_b;           /* atransform/btransform == conj         */
+  int32_t memory;                   /* qs_memory, shared by A, B and C       */
+  int32_t precision;                /* qs_precision                          */
+  qs_group m, n, k;
+  const void *a; int64_t a_base; int64_t a_len;
+  const void *b; int64_t b_base; int64_t b_len;
+  void *c;       int64_t c_base; int64_t c_len;
+  double target_rel_err;            /* split precision: error to meet (0: default) */
+  int32_t tile;                     /* tile-config override, -1 = automatic  */
+  int32_t path_mode;                /* 0 auto; 1 force the tiled MFMA GEMM path */
+} qs_contract_desc;
+
+typedef struct qs_plan qs_plan;
+
+/* The execution path a plan runs (src/execution/paths.jl:1-74 counterparts). */
+typedef enum qs_path {
+  QS_PATH_EMPTY = 0,   /* M or N empty: nothing to do                    */
+  QS_PATH_SCALE = 1,   /* K empty: C = beta * C                          */
+  QS_PATH_GEMM = 2,    /* tiled MFMA block-scatter GEMM, alpha/beta epilogue */
+  QS_PATH_DOT = 3,     /* M == 1 or N == 1: bandwidth-bound reduction     */
+  QS_PATH_OUTER = 4,   /* K == 1: bandwidth-bound streaming outer product */
+  QS_PATH_SKINNY = 5,  /* small M: B streamed once, read in place         */
+  QS_PATH_SPLIT = 6    /* split-precision (INT8 slices) GEMM              */
+} qs_path;
+
+/* What a plan decided (for tests, logs and benchmarks). */
+typedef struct qs_plan_info {
+  int32_t path;          /* qs_path */
+  int32_t tile_m, tile_n, tile_k;
+  int32_t k_splits;      /* K partitions reduced in a fixed order (1 = none) */
+  int32_t a_kfast, b_kfast;  /* operand gathers coalesced along K (1) or along M/N (0) */
+  int32_t slices;        /* split precision: INT8 slices per operand      */
+  int64_t m, n, k;       /* axis lengths */
+  int64_t workspace_bytes;
+} qs_plan_info;
+
+/* The gfx950 TargetProfile (counterpart of src/hardware/target.jl:26-47). */
+typedef struct qs_target_profile {
+  char name[64];            /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+  int32_t compute_units;
+  int32_t wavefront_size;
+  int32_t lds_bytes_per_cu;
+  int32_t lds_bytes_per_block;
+  int32_t clock_khz;
+  int32_t memory_clock_khz;
+  int32_t memory_bus_bits;
+  int32_t l2_bytes;
+  int64_t hbm_bytes;
+  int32_t xcds;
+  int32_t mfma_f64_m, mfma_f64_n, mfma_f64_k;   /* 16x16x4  */
+  int32_t mfma_f32_m, mfma_f32_n, mfma_f32_k;   /* 16x16x4  */
+  int32_t mfma_i8_m, mfma_i8_n, mfma_i8_k;      /* 16x16x64 */
+  double peak_fp64_tflops, peak_fp32_tflops, peak_int8_tops, peak_hbm_gbs;
+} qs_target_profile;
+
+/* Last error message of the calling thread ("" if none). */
+const char *qs_last_error(void);
+
+/* Library version string. */
+const char *qs_version(void);
+
+/* Fill `out` from hipGetDeviceProperties of the current device plus the
+ * gfx950 MFMA table.  Counterpart of `target_profile()`
+ * (src/hardware/target.jl:181-191); never fails on a gfx950 device. */
+qs_status qs_target_profile_get(qs_target_profile *out);
+
+/* Validate `desc` and build a plan: the axis-group overflow policy
+ * (axis_group.jl:27-41), eltype/compute-type rules (plan.jl:278-295), offset
+ * tables (fill_offsets!, axis_group.jl:87-145) uploaded once, the exact span of
+ * every operand (tiles.jl:123-134), and the path/tile choice.
+ * Counterpart of `plan_contract` after label resolution (plan.jl:115-133). */
+qs_status qs_plan_create(const qs_contract_desc *desc, qs_plan **out);
+
+/* C = alpha * sum_K op(A) op(B) + beta * C.  alpha and beta point to values of
+ * the compute type (complex: two doubles or floats, re then im).  beta == 0
+ * never reads C; alpha == 0 or K == 0 never reads A or B; beta is applied
+ * exactly once.  Operand spans are checked before any launch (BoundsError
+ * with nothing written).  Counterpart of `execute!` (execute.jl:11-20). */
+qs_status qs_execute(qs_plan *plan, const void *alpha, const void *beta, void *stream);
+
+/* Plan introspection. */
+qs_status qs_plan_get_info(const qs_plan *plan, qs_plan_info *out);
+
+/* Point an existing plan at other storages of the same layout (device
+ * memory only); re-runs the span check.  Lets one plan serve many requests. */
+qs_status qs_plan_rebind(qs_plan *plan, const void *a, int64_t a_base, int64_t a_len,
+                         const void *b, int64_t b_base, int64_t b_len,
+                         void *c, int64_t c_base, int64_t c_len);
+
+/* Free a plan and its device tables.  Counterpart of `release!`
+ * (src/execution/workspace.jl:110-123).  NULL is a no-op. */
+qs_status qs_plan_destroy(qs_plan *plan);
+
+/* Host-side offset enumeration, exactly `fill_offsets!`
+ * (src/layout/axis_group.jl:87-145) for a two-map group: writes `count`
+ * offsets of coordinates [first, first+count) into out0/out1.  Exposed so a
+ * caller can check a group against its own indexing. */
+qs_status qs_fill_offsets(const qs_group *g, int64_t first, int64_t count,
+                          int64_t *out0, int64_t *out1);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QS_AMD_H */

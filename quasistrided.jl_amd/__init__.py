@@ -1,0 +1,26 @@
+"""MI355X (gfx950) block-scatter tensor contraction: a drop-in engine for
+QuasiStrided.jl's `plan_contract` / `execute!` / `contract!` /
+`QuasiStridedBackend` path.
+
+Host planning (labels -> M/N/K axis groups) is done here as the reference
+does it; the contraction runs in libqs_amd.so, hand-written HIP kernels for
+gfx950 behind the C ABI of include/qs_amd.h.
+"""
+from .errors import ArgumentError, BoundsError, DeviceError, DimensionMismatch, UnsupportedError
+from .strided import StridedView
+from .labels import (AxisGroup, affine_ramp, axis_length, classify_labels, group_ranks, leading_unit_run,
+                     map_ramp_step, order_contract_labels, order_free_labels, prefer_swap)
+from .plan import (ContractPlan, MFMAKernel, PathModes, compute_type, contract, contract_, execute, execute_,
+                   isconj, op_conjugates, plan_contract, planned, release)
+from .backend import QuasiStridedBackend, contraction_labels, tensorcontract
+from .target import target_profile
+from . import _lib
+
+__all__ = [
+    "ArgumentError", "BoundsError", "DeviceError", "DimensionMismatch", "UnsupportedError", "StridedView",
+    "AxisGroup", "affine_ramp", "axis_length", "classify_labels", "group_ranks", "leading_unit_run",
+    "map_ramp_step", "order_contract_labels", "order_free_labels", "prefer_swap", "ContractPlan", "MFMAKernel",
+    "PathModes", "compute_type", "contract", "contract_", "execute", "execute_", "isconj", "op_conjugates",
+    "plan_contract", "planned", "release", "QuasiStridedBackend", "contraction_labels", "tensorcontract",
+    "target_profile",
+]

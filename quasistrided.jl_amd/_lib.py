@@ -1,0 +1,123 @@
+"""ctypes binding of libqs_amd.so (include/qs_amd.h).
+
+The library is loaded from this package directory only; a missing library is a
+hard error (there is no CPU fallback anywhere in the product).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqs_amd.so")
+
+QS_MAX_DIMS = 16
+
+QS_OK, QS_ERR_ARGUMENT, QS_ERR_DIMENSION, QS_ERR_BOUNDS, QS_ERR_OVERFLOW = 0, 1, 2, 3, 4
+QS_ERR_DEVICE, QS_ERR_MEMORY, QS_ERR_UNSUPPORTED = 5, 6, 7
+
+QS_F32, QS_F64, QS_C32, QS_C64 = 0, 1, 2, 3
+QS_MEM_DEVICE, QS_MEM_HOST = 0, 1
+QS_PLAIN, QS_SPLIT_INT8 = 0, 1
+
+PATH_NAMES = {0: "empty", 1: "scale", 2: "gemm", 3: "dot", 4: "outer", 5: "skinny", 6: "split"}
+
+
+class QSGroup(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int32),
+        ("_pad", C.c_int32),
+        ("lengths", C.c_int64 * QS_MAX_DIMS),
+        ("strides", (C.c_int64 * QS_MAX_DIMS) * 2),
+    ]
+
+
+class QSContractDesc(C.Structure):
+    _fields_ = [
+        ("a_type", C.c_int32), ("b_type", C.c_int32), ("c_type", C.c_int32),
+        ("compute_type", C.c_int32),
+        ("conj_a", C.c_int32), ("conj_b", C.c_int32),
+        ("memory", C.c_int32), ("precision", C.c_int32),
+        ("m", QSGroup), ("n", QSGroup), ("k", QSGroup),
+        ("a", C.c_void_p), ("a_base", C.c_int64), ("a_len", C.c_int64),
+        ("b", C.c_void_p), ("b_base", C.c_int64), ("b_len", C.c_int64),
+        ("c", C.c_void_p), ("c_base", C.c_int64), ("c_len", C.c_int64),
+        ("target_rel_err", C.c_double),
+        ("tile", C.c_int32),
+        ("path_mode", C.c_int32),
+    ]
+
+
+class QSPlanInfo(C.Structure):
+    _fields_ = [
+        ("path", C.c_int32), ("tile_m", C.c_int32), ("tile_n", C.c_int32), ("tile_k", C.c_int32),
+        ("k_splits", C.c_int32), ("a_kfast", C.c_int32), ("b_kfast", C.c_int32), ("slices", C.c_int32),
+        ("m", C.c_int64), ("n", C.c_int64), ("k", C.c_int64), ("workspace_bytes", C.c_int64),
+    ]
+
+
+class QSTargetProfile(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 64),
+        ("compute_units", C.c_int32), ("wavefront_size", C.c_int32),
+        ("lds_bytes_per_cu", C.c_int32), ("lds_bytes_per_block", C.c_int32),
+        ("clock_khz", C.c_int32), ("memory_clock_khz", C.c_int32), ("memory_bus_bits", C.c_int32),
+        ("l2_bytes", C.c_int32), ("hbm_bytes", C.c_int64), ("xcds", C.c_int32),
+        ("mfma_f64_m", C.c_int32), ("mfma_f64_n", C.c_int32), ("mfma_f64_k", C.c_int32),
+        ("mfma_f32_m", C.c_int32), ("mfma_f32_n", C.c_int32), ("mfma_f32_k", C.c_int32),
+        ("mfma_i8_m", C.c_int32), ("mfma_i8_n", C.c_int32), ("mfma_i8_k", C.c_int32),
+        ("peak_fp64_tflops", C.c_double), ("peak_fp32_tflops", C.c_double),
+        ("peak_int8_tops", C.c_double), ("peak_hbm_gbs", C.c_double),
+    ]
+
+
+# Every symbol include/qs_amd.h declares: name -> (restype, argtypes).
+SIGNATURES = {
+    "qs_last_error": (C.c_char_p, []),
+    "qs_version": (C.c_char_p, []),
+    "qs_target_profile_get": (C.c_int, [C.POINTER(QSTargetProfile)]),
+    "qs_plan_create": (C.c_int, [C.POINTER(QSContractDesc), C.POINTER(C.c_void_p)]),
+    "qs_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "qs_plan_get_info": (C.c_int, [C.c_void_p, C.POINTER(QSPlanInfo)]),
+    "qs_plan_rebind": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
+                                 C.c_void_p, C.c_int64, C.c_int64]),
+    "qs_plan_destroy": (C.c_int, [C.c_void_p]),
+    "qs_fill_offsets": (C.c_int, [C.POINTER(QSGroup), C.c_int64, C.c_int64, C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libqs_amd.so not found at {LIB_PATH}: build it first "
+                "(python quasistrided.jl_amd/build.py or __graft_entry__.build())")
+        handle = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    return lib().qs_last_error().decode()
+
+
+def make_group(lengths, strides0, strides1) -> QSGroup:
+    g = QSGroup()
+    D = len(lengths)
+    if D > QS_MAX_DIMS:
+        raise ValueError(f"a group of {D} dims exceeds the library limit of {QS_MAX_DIMS}")
+    g.ndim = D
+    for d in range(D):
+        g.lengths[d] = int(lengths[d])
+        g.strides[0][d] = int(strides0[d])
+        g.strides[1][d] = int(strides1[d])
+    return g

@@ -1,0 +1,537 @@
+// The C ABI (include/qs_amd.h): validation, plans, path and tile choice,
+// launches.  Host code only; kernels live in the .hip files.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "qs_internal.h"
+#include "qs_plan.h"
+
+namespace qs {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+qs_status fail(qs_status code, const std::string &msg) {
+  set_error(msg);
+  return code;
+}
+
+static qs_status hip_fail(hipError_t e, const char *what) {
+  return fail(QS_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ groups
+
+int64_t group_length(const qs_group &g) {
+  int64_t q = 1;
+  for (int d = 0; d < g.ndim; ++d) q *= g.lengths[d];
+  return q;
+}
+
+// AxisGroup validation, src/layout/axis_group.jl:9-41.
+qs_status validate_group(const qs_group &g, const char *name) {
+  if (g.ndim < 0 || g.ndim > QS_MAX_DIMS)
+    return fail(QS_ERR_ARGUMENT, std::string(name) + " group: ndim out of range [0, 16]");
+  for (int d = 0; d < g.ndim; ++d)
+    if (g.lengths[d] < 0)
+      return fail(QS_ERR_ARGUMENT, std::string(name) + " group: AxisGroup lengths must be nonnegative, got lengths[" +
+                                       std::to_string(d + 1) + "] = " + std::to_string(g.lengths[d]));
+  for (int d = 0; d < g.ndim; ++d)
+    if (g.lengths[d] == 0) return QS_OK;
+  int64_t card = 1;
+  for (int d = 0; d < g.ndim; ++d)
+    if (__builtin_mul_overflow(card, g.lengths[d], &card))
+      return fail(QS_ERR_OVERFLOW, std::string(name) + " group: cardinality overflows Int64");
+  for (int p = 0; p < 2; ++p) {
+    int64_t acc = 0;
+    for (int d = 0; d < g.ndim; ++d) {
+      if (g.lengths[d] <= 1) continue;
+      const int64_t s = g.strides[p][d];
+      if (s == INT64_MIN) return fail(QS_ERR_OVERFLOW, std::string(name) + " group: |stride| overflows Int64");
+      int64_t term;
+      if (__builtin_mul_overflow(g.lengths[d] - 1, s < 0 ? -s : s, &term) || __builtin_add_overflow(acc, term, &acc))
+        return fail(QS_ERR_OVERFLOW, std::string(name) + " group: offset range overflows Int64");
+    }
+  }
+  return QS_OK;
+}
+
+// fill_offsets! (axis_group.jl:87-145) for both maps of a two-map group.
+void fill_offsets(const qs_group &g, int64_t first, int64_t count, int64_t *o0, int64_t *o1) {
+  if (count <= 0) return;
+  int64_t x[QS_MAX_DIMS] = {0};
+  int64_t a = 0, b = 0, r = first;
+  for (int d = 0; d < g.ndim; ++d) {
+    const int64_t L = g.lengths[d];
+    x[d] = r % L;
+    r /= L;
+    a += x[d] * g.strides[0][d];
+    b += x[d] * g.strides[1][d];
+  }
+  for (int64_t t = 0;;) {
+    o0[t] = a;
+    o1[t] = b;
+    if (++t == count) break;
+    for (int d = 0; d < g.ndim; ++d) {
+      const int64_t L = g.lengths[d];
+      if (x[d] < L - 1) {
+        ++x[d];
+        a += g.strides[0][d];
+        b += g.strides[1][d];
+        break;
+      }
+      x[d] = 0;
+      a -= (L - 1) * g.strides[0][d];
+      b -= (L - 1) * g.strides[1][d];
+    }
+  }
+}
+
+// Exact offset range of map p (per dim min(0,(L-1)s) / max(0,(L-1)s)).
+void map_range(const qs_group &g, int p, int64_t &lo, int64_t &hi) {
+  lo = hi = 0;
+  for (int d = 0; d < g.ndim; ++d) {
+    if (g.lengths[d] <= 1) continue;
+    const int64_t e = (g.lengths[d] - 1) * g.strides[p][d];
+    if (e < 0) lo += e; else hi += e;
+  }
+}
+
+// Move dim `d` of group `g` to the front (the enumeration order changes; the
+// set of (map0, map1) offset pairs does not).
+static void move_to_front(qs_group &g, int d) {
+  if (d <= 0) return;
+  const int64_t L = g.lengths[d], s0 = g.strides[0][d], s1 = g.strides[1][d];
+  for (int e = d; e > 0; --e) {
+    g.lengths[e] = g.lengths[e - 1];
+    g.strides[0][e] = g.strides[0][e - 1];
+    g.strides[1][e] = g.strides[1][e - 1];
+  }
+  g.lengths[0] = L;
+  g.strides[0][0] = s0;
+  g.strides[1][0] = s1;
+}
+
+// A non-singleton dim of map p with |stride| == 1, or -1.
+static int unit_dim(const qs_group &g, int p) {
+  for (int d = 0; d < g.ndim; ++d)
+    if (g.lengths[d] > 1 && (g.strides[p][d] == 1 || g.strides[p][d] == -1)) return d;
+  return -1;
+}
+
+// ------------------------------------------------------------------ tiles
+
+static int pick_bk_host(int bm, int bn, int ct) {
+  const int planes = is_complex(ct) ? 2 : 1;
+  const int rsize = is_double(ct) ? 8 : 4;
+  const int bk = (48 * 1024) / (planes * (bm + bn + 32) * rsize * 2);
+  int p = 4;
+  while (p * 2 <= bk && p * 2 <= 32) p *= 2;
+  return p;
+}
+
+TileShape tile_shape(int tile, int ct) {
+  int bm = 64, bn = 64;
+  switch (tile) {
+    case TILE_128x128: bm = 128; bn = 128; break;
+    case TILE_128x64: bm = 128; bn = 64; break;
+    case TILE_64x128: bm = 64; bn = 128; break;
+    case TILE_32x64: bm = 32; bn = 64; break;
+    default: break;
+  }
+  return TileShape{bm, bn, pick_bk_host(bm, bn, ct)};
+}
+
+static int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Workgroup tile for an M x N x K contraction of compute type ct: enough
+// tiles to cover the 256 CUs, the largest register tile that does so.
+static int choose_tile(int64_t M, int64_t N, int ct) {
+  const bool cplx = is_complex(ct);
+  if (M <= 32) return TILE_32x64;
+  const int64_t t128 = cdiv(M, 128) * cdiv(N, 128);
+  if (!cplx && t128 >= 256) return TILE_128x128;
+  const int64_t t12864 = cdiv(M, 128) * cdiv(N, 64);
+  if (t12864 >= 256) return TILE_128x64;
+  return TILE_64x64;
+}
+
+// Fixed-order split-K when the tile grid alone leaves CUs idle.
+static int choose_splits(int64_t tiles, int64_t K, int bk, int64_t &k_chunk) {
+  int s = 1;
+  if (tiles < 256 && K >= 512) {
+    s = (int)std::min<int64_t>(cdiv(512, tiles), K / 256);
+    s = std::max(s, 1);
+    s = std::min(s, 16);
+  }
+  k_chunk = cdiv(cdiv(K, s), bk) * bk;
+  s = (int)cdiv(K, k_chunk);
+  return s;
+}
+
+// ------------------------------------------------------------------- plan
+
+static int bytes_of_compute_real(int ct) { return is_double(ct) ? 8 : 4; }
+
+static double scalar_at(const void *p, int ct, int part) {
+  if (ct == QS_F32 || ct == QS_C32) return (double)((const float *)p)[part];
+  return ((const double *)p)[part];
+}
+
+qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
+  if (!desc || !out) return fail(QS_ERR_ARGUMENT, "qs_plan_create: null argument");
+  const qs_contract_desc &d = *desc;
+  for (int t : {d.a_type, d.b_type, d.c_type, d.compute_type})
+    if (t < QS_F32 || t > QS_C64)
+      return fail(QS_ERR_ARGUMENT, "plan_contract: eltypes must be Float32, Float64, ComplexF32 or ComplexF64");
+  if (!is_complex(d.c_type) && (is_complex(d.a_type) || is_complex(d.b_type)))
+    return fail(QS_ERR_ARGUMENT, "plan_contract: a complex operand needs a complex C");
+  if (is_complex(d.compute_type) != (is_complex(d.a_type) || is_complex(d.b_type) || is_complex(d.c_type)))
+    return fail(QS_ERR_ARGUMENT, "plan_contract: the compute type's domain must be the promoted domain of A, B, C");
+  if (d.memory != QS_MEM_DEVICE && d.memory != QS_MEM_HOST)
+    return fail(QS_ERR_ARGUMENT, "plan_contract: memory must be QS_MEM_DEVICE or QS_MEM_HOST");
+  if (d.precision != QS_PLAIN && d.precision != QS_SPLIT_INT8)
+    return fail(QS_ERR_ARGUMENT, "plan_contract: unknown precision mode");
+  qs_status st;
+  if ((st = validate_group(d.m, "M")) != QS_OK) return st;
+  if ((st = validate_group(d.n, "N")) != QS_OK) return st;
+  if ((st = validate_group(d.k, "K")) != QS_OK) return st;
+  if (d.a_len < 0 || d.b_len < 0 || d.c_len < 0)
+    return fail(QS_ERR_ARGUMENT, "plan_contract: negative storage length");
+
+  qs_plan *p = new qs_plan();
+  p->desc = d;
+  p->M = group_length(d.m);
+  p->N = group_length(d.n);
+  p->K = group_length(d.k);
+  p->mg = d.m;
+  p->ng = d.n;
+  p->kg = d.k;
+
+  // Spans for the exact bounds checks (tiles.jl:120-134), in the caller's
+  // group order (the ranges do not depend on the order).
+  map_range(d.m, 0, p->mA_lo, p->mA_hi);
+  map_range(d.m, 1, p->mC_lo, p->mC_hi);
+  map_range(d.n, 0, p->nB_lo, p->nB_hi);
+  map_range(d.n, 1, p->nC_lo, p->nC_hi);
+  map_range(d.k, 0, p->kA_lo, p->kA_hi);
+  map_range(d.k, 1, p->kB_lo, p->kB_hi);
+
+  const int ct = d.compute_type;
+  p->mixed = !(d.a_type == ct && d.b_type == ct && d.c_type == ct);
+
+  // Path choice (select_path, plan.jl:395-412, with GPU paths).
+  const int64_t M = p->M, N = p->N, K = p->K;
+  if (M == 0 || N == 0) p->path = QS_PATH_EMPTY;
+  else if (K == 0) p->path = QS_PATH_SCALE;
+  else if (d.path_mode == 1) p->path = QS_PATH_GEMM;
+  else if ((M == 1 || N == 1)) p->path = QS_PATH_DOT;
+  else if (K == 1) p->path = QS_PATH_OUTER;
+  else if (M <= 32) p->path = QS_PATH_SKINNY;
+  else p->path = QS_PATH_GEMM;
+  if (d.precision == QS_SPLIT_INT8 && (p->path == QS_PATH_GEMM || p->path == QS_PATH_SKINNY)) {
+    delete p;
+    return fail(QS_ERR_UNSUPPORTED, "plan_contract: split precision is not available in this build");
+  }
+
+  // Coalescing: put a unit-stride dim first in each operand's gather
+  // direction.  Reordering M or N dims changes no arithmetic; reordering K
+  // changes only the summation order.
+  qs_group &mg = p->mg, &ng = p->ng, &kg = p->kg;
+  if (p->path == QS_PATH_OUTER) {
+    const int cm = unit_dim(mg, 1), cn = unit_dim(ng, 1);
+    if (cm >= 0) move_to_front(mg, cm);
+    else if (cn >= 0) move_to_front(ng, cn);
+    p->a_kfast = (cm < 0 && cn >= 0) ? 1 : 0;   // "N fastest" for the outer kernel
+    p->b_kfast = 0;
+  } else {
+    const int am = unit_dim(mg, 0), ak = unit_dim(kg, 0);
+    const int bn = unit_dim(ng, 0), bk = unit_dim(kg, 1);
+    bool akf = am < 0 && ak >= 0, bkf = bn < 0 && bk >= 0;
+    if (p->path == QS_PATH_DOT) {   // only the matrix operand's choice matters
+      if (N == 1) { bkf = false; } else { akf = false; }
+    }
+    if (akf && bkf && ak != bk) {
+      // Both operands want a different K dim first: the larger one wins.
+      if (M >= N) bkf = false; else akf = false;
+    }
+    if (am >= 0) move_to_front(mg, am);
+    if (bn >= 0) move_to_front(ng, bn);
+    if (akf) move_to_front(kg, ak);
+    else if (bkf) move_to_front(kg, bk);
+    p->a_kfast = akf;
+    p->b_kfast = bkf;
+  }
+
+  // Offset tables, built once and kept on the device.
+  const int64_t nt = 2 * (M + N + K);
+  std::vector<int64_t> h((size_t)std::max<int64_t>(nt, 1));
+  int64_t *mA = h.data(), *mC = mA + M, *nB = mC + M, *nC = nB + N, *kA = nC + N, *kB = kA + K;
+  fill_offsets(mg, 0, M, mA, mC);
+  fill_offsets(ng, 0, N, nB, nC);
+  fill_offsets(kg, 0, K, kA, kB);
+  hipError_t e = hipSuccess;
+  if (nt > 0) {
+    e = hipMalloc(&p->d_tables, sizeof(int64_t) * (size_t)nt);
+    if (e == hipSuccess) e = hipMemcpy(p->d_tables, h.data(), sizeof(int64_t) * (size_t)nt, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      qs_status r = e == hipErrorOutOfMemory ? fail(QS_ERR_MEMORY, "plan_contract: device tables") : hip_fail(e, "plan_contract: tables");
+      qs_plan_destroy(p);
+      return r;
+    }
+  }
+  p->t_mA = p->d_tables;
+  p->t_mC = p->t_mA + M;
+  p->t_nB = p->t_mC + M;
+  p->t_nC = p->t_nB + N;
+  p->t_kA = p->t_nC + N;
+  p->t_kB = p->t_kA + K;
+
+  // Tile, splits, workspace.
+  p->tile = TILE_64x64;
+  p->k_splits = 1;
+  p->k_chunk = K;
+  if (p->path == QS_PATH_GEMM || p->path == QS_PATH_SKINNY) {
+    p->tile = (d.tile >= 0 && d.tile < TILE_COUNT) ? d.tile : choose_tile(M, N, ct);
+    const TileShape ts = tile_shape(p->tile, ct);
+    const int64_t tiles = cdiv(M, ts.bm) * cdiv(N, ts.bn);
+    if (tiles > INT32_MAX) { qs_plan_destroy(p); return fail(QS_ERR_UNSUPPORTED, "plan_contract: too many tiles"); }
+    p->k_splits = choose_splits(tiles, K, ts.bk, p->k_chunk);
+    if (p->k_splits > 1) {
+      const size_t bytes = (size_t)p->k_splits * (is_complex(ct) ? 2 : 1) * (size_t)M * (size_t)N * bytes_of_compute_real(ct);
+      e = hipMalloc(&p->d_partials, bytes);
+      if (e != hipSuccess) {
+        // No room for partial sums: run unsplit.
+        p->d_partials = nullptr;
+        p->k_splits = 1;
+        p->k_chunk = K;
+        (void)hipGetLastError();
+      } else {
+        p->workspace_bytes = (int64_t)bytes;
+      }
+    }
+  }
+  *out = p;
+  return QS_OK;
+}
+
+static bool span_ok(int64_t base, int64_t rlo, int64_t rhi, int64_t clo, int64_t chi, int64_t len, int64_t &lo_out,
+                    int64_t &hi_out) {
+  const __int128 lo = (__int128)base + rlo + clo, hi = (__int128)base + rhi + chi;
+  lo_out = (int64_t)lo;
+  hi_out = (int64_t)hi;
+  return lo >= 0 && hi <= (__int128)len - 1;
+}
+
+static qs_status check_spans(qs_plan *p, bool reads_ab) {
+  const qs_contract_desc &d = p->desc;
+  if (p->M == 0 || p->N == 0) return QS_OK;
+  if (!span_ok(d.c_base, p->mC_lo, p->mC_hi, p->nC_lo, p->nC_hi, d.c_len, p->c_lo, p->c_hi))
+    return fail(QS_ERR_BOUNDS, "execute!: C tile addresses [" + std::to_string(p->c_lo) + ", " + std::to_string(p->c_hi) +
+                                   "] exceed storage bounds [0, " + std::to_string(d.c_len - 1) + "]");
+  if (reads_ab && p->K > 0) {
+    if (!span_ok(d.a_base, p->mA_lo, p->mA_hi, p->kA_lo, p->kA_hi, d.a_len, p->a_lo, p->a_hi))
+      return fail(QS_ERR_BOUNDS, "execute!: A tile addresses [" + std::to_string(p->a_lo) + ", " +
+                                     std::to_string(p->a_hi) + "] exceed storage bounds [0, " +
+                                     std::to_string(d.a_len - 1) + "]");
+    if (!span_ok(d.b_base, p->kB_lo, p->kB_hi, p->nB_lo, p->nB_hi, d.b_len, p->b_lo, p->b_hi))
+      return fail(QS_ERR_BOUNDS, "execute!: B tile addresses [" + std::to_string(p->b_lo) + ", " +
+                                     std::to_string(p->b_hi) + "] exceed storage bounds [0, " +
+                                     std::to_string(d.b_len - 1) + "]");
+  }
+  return QS_OK;
+}
+
+GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *B, int64_t b_base, void *C,
+                   int64_t c_base) {
+  const qs_contract_desc &d = p->desc;
+  GemmArgs g;
+  g.A = A;
+  g.B = B;
+  g.C = C;
+  g.a_base = a_base;
+  g.b_base = b_base;
+  g.c_base = c_base;
+  g.mA = p->t_mA;
+  g.kA = p->t_kA;
+  g.nB = p->t_nB;
+  g.kB = p->t_kB;
+  g.mC = p->t_mC;
+  g.nC = p->t_nC;
+  g.M = p->M;
+  g.N = p->N;
+  g.K = p->K;
+  g.a_type = d.a_type;
+  g.b_type = d.b_type;
+  g.c_type = d.c_type;
+  g.compute_type = d.compute_type;
+  g.conj_a = is_complex(d.compute_type) ? d.conj_a : 0;
+  g.conj_b = is_complex(d.compute_type) ? d.conj_b : 0;
+  g.alpha_re = 1;
+  g.alpha_im = 0;
+  g.beta_re = 0;
+  g.beta_im = 0;
+  g.beta_case = 0;
+  g.k_splits = p->k_splits;
+  g.k_chunk = p->k_chunk;
+  g.partials = p->d_partials;
+  g.tile = p->tile;
+  g.a_kfast = p->a_kfast;
+  g.b_kfast = p->b_kfast;
+  g.mixed = p->mixed;
+  return g;
+}
+
+// Run the plan's path on device storages (bases already shifted for staging).
+qs_status run_device(qs_plan *p, const void *A, int64_t a_base, const void *B, int64_t b_base, void *C, int64_t c_base,
+                     const double *al, const double *be, hipStream_t s) {
+  GemmArgs g = make_args(p, A, a_base, B, b_base, C, c_base);
+  g.alpha_re = al[0];
+  g.alpha_im = al[1];
+  g.beta_re = be[0];
+  g.beta_im = be[1];
+  g.beta_case = (be[0] == 0 && be[1] == 0) ? 0 : (be[0] == 1 && be[1] == 0) ? 1 : 2;
+  const bool alpha_zero = al[0] == 0 && al[1] == 0;
+  hipError_t e = hipSuccess;
+  if (p->path == QS_PATH_EMPTY) return QS_OK;
+  if (alpha_zero || p->path == QS_PATH_SCALE) {
+    if (g.beta_case == 1) return QS_OK;   // scale_tile!: beta == 1 is a no-op
+    e = launch_scale(g, s);
+  } else if (p->path == QS_PATH_DOT) {
+    e = launch_dot(g, s);
+  } else if (p->path == QS_PATH_OUTER) {
+    e = launch_outer(g, s);
+  } else {
+    e = launch_gemm(g, s);
+  }
+  if (e != hipSuccess) return hip_fail(e, "execute!: launch");
+  return QS_OK;
+}
+
+}  // namespace qs
+
+using namespace qs;
+
+extern "C" {
+
+const char *qs_last_error(void) { return g_last_error.c_str(); }
+
+const char *qs_version(void) { return "qs_amd 0.1.0 (gfx950)"; }
+
+qs_status qs_target_profile_get(qs_target_profile *out) {
+  if (!out) return fail(QS_ERR_ARGUMENT, "qs_target_profile_get: null argument");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  hipDeviceProp_t pr;
+  e = hipGetDeviceProperties(&pr, dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+  std::memset(out, 0, sizeof *out);
+  std::snprintf(out->name, sizeof out->name, "%s", pr.gcnArchName);
+  out->compute_units = pr.multiProcessorCount;
+  out->wavefront_size = pr.warpSize;
+  out->lds_bytes_per_cu = (int32_t)pr.maxSharedMemoryPerMultiProcessor;
+  out->lds_bytes_per_block = (int32_t)pr.sharedMemPerBlock;
+  out->clock_khz = pr.clockRate;
+  out->memory_clock_khz = pr.memoryClockRate;
+  out->memory_bus_bits = pr.memoryBusWidth;
+  out->l2_bytes = pr.l2CacheSize;
+  out->hbm_bytes = (int64_t)pr.totalGlobalMem;
+  out->xcds = 8;
+  out->mfma_f64_m = 16; out->mfma_f64_n = 16; out->mfma_f64_k = 4;
+  out->mfma_f32_m = 16; out->mfma_f32_n = 16; out->mfma_f32_k = 4;
+  out->mfma_i8_m = 16; out->mfma_i8_n = 16; out->mfma_i8_k = 64;
+  out->peak_fp64_tflops = 78.6;
+  out->peak_fp32_tflops = 157.3;
+  out->peak_int8_tops = 5000.0;
+  out->peak_hbm_gbs = 8000.0;
+  return QS_OK;
+}
+
+qs_status qs_plan_create(const qs_contract_desc *desc, qs_plan **out) {
+  if (out) *out = nullptr;
+  return plan_build(desc, out);
+}
+
+qs_status qs_plan_get_info(const qs_plan *p, qs_plan_info *out) {
+  if (!p || !out) return fail(QS_ERR_ARGUMENT, "qs_plan_get_info: null argument");
+  std::memset(out, 0, sizeof *out);
+  out->path = p->path;
+  const TileShape ts = tile_shape(p->tile, p->desc.compute_type);
+  out->tile_m = ts.bm;
+  out->tile_n = ts.bn;
+  out->tile_k = ts.bk;
+  out->k_splits = p->k_splits;
+  out->a_kfast = p->a_kfast;
+  out->b_kfast = p->b_kfast;
+  out->slices = p->slices;
+  out->m = p->M;
+  out->n = p->N;
+  out->k = p->K;
+  out->workspace_bytes = p->workspace_bytes;
+  return QS_OK;
+}
+
+qs_status qs_execute(qs_plan *p, const void *alpha, const void *beta, void *stream) {
+  if (!p || !alpha || !beta) return fail(QS_ERR_ARGUMENT, "execute!: null argument");
+  const int ct = p->desc.compute_type;
+  double al[2] = {scalar_at(alpha, ct, 0), is_complex(ct) ? scalar_at(alpha, ct, 1) : 0.0};
+  double be[2] = {scalar_at(beta, ct, 0), is_complex(ct) ? scalar_at(beta, ct, 1) : 0.0};
+  const bool reads_ab = !(al[0] == 0 && al[1] == 0);
+  qs_status st = check_spans(p, reads_ab);
+  if (st != QS_OK) return st;
+  if (p->path == QS_PATH_EMPTY) return QS_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const qs_contract_desc &d = p->desc;
+  if (d.memory == QS_MEM_DEVICE)
+    return run_device(p, d.a, d.a_base, d.b, d.b_base, d.c, d.c_base, al, be, s);
+  return run_staged(p, al, be, reads_ab, s);
+}
+
+qs_status qs_plan_rebind(qs_plan *p, const void *a, int64_t a_base, int64_t a_len, const void *b, int64_t b_base,
+                         int64_t b_len, void *c, int64_t c_base, int64_t c_len) {
+  if (!p) return fail(QS_ERR_ARGUMENT, "qs_plan_rebind: null plan");
+  if (p->desc.memory != QS_MEM_DEVICE) return fail(QS_ERR_ARGUMENT, "qs_plan_rebind: device plans only");
+  if (a_len < 0 || b_len < 0 || c_len < 0) return fail(QS_ERR_ARGUMENT, "qs_plan_rebind: negative storage length");
+  p->desc.a = a; p->desc.a_base = a_base; p->desc.a_len = a_len;
+  p->desc.b = b; p->desc.b_base = b_base; p->desc.b_len = b_len;
+  p->desc.c = c; p->desc.c_base = c_base; p->desc.c_len = c_len;
+  return QS_OK;
+}
+
+qs_status qs_plan_destroy(qs_plan *p) {
+  if (!p) return QS_OK;
+  if (p->d_tables) (void)hipFree(p->d_tables);
+  if (p->d_partials) (void)hipFree(p->d_partials);
+  for (int i = 0; i < 3; ++i)
+    if (p->stage[i]) (void)hipFree(p->stage[i]);
+  delete p;
+  return QS_OK;
+}
+
+qs_status qs_fill_offsets(const qs_group *g, int64_t first, int64_t count, int64_t *out0, int64_t *out1) {
+  if (!g) return fail(QS_ERR_ARGUMENT, "fill_offsets!: null group");
+  qs_status st = validate_group(*g, "fill_offsets!");
+  if (st != QS_OK) return st;
+  const int64_t Q = group_length(*g);
+  if (count < 0) return fail(QS_ERR_ARGUMENT, "count must be nonnegative, got " + std::to_string(count));
+  if (!(0 <= first && first <= Q))
+    return fail(QS_ERR_BOUNDS, "AxisGroup interval start " + std::to_string(first) + " out of range [0, " +
+                                   std::to_string(Q) + "]");
+  if (count > Q - first)
+    return fail(QS_ERR_BOUNDS, "AxisGroup interval (first=" + std::to_string(first) + ", count=" +
+                                   std::to_string(count) + ") exceeds domain size " + std::to_string(Q));
+  if (count > 0 && (!out0 || !out1)) return fail(QS_ERR_DIMENSION, "fill_offsets!: null buffer");
+  fill_offsets(*g, first, count, out0, out1);
+  return QS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// Internal declarations shared by the C-ABI layer (qs_api.cpp) and the
+// gfx950 kernels (qs_gemm.hip, qs_bw.hip).  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/qs_amd.h"
+
+namespace qs {
+
+// Thread-local error text behind qs_last_error().
+void set_error(const std::string &msg);
+qs_status fail(qs_status code, const std::string &msg);
+
+inline bool is_complex(int t) { return t == QS_C32 || t == QS_C64; }
+inline bool is_double(int t) { return t == QS_F64 || t == QS_C64; }
+inline int elem_bytes(int t) { return t == QS_F32 ? 4 : (t == QS_F64 || t == QS_C32) ? 8 : 16; }
+
+// Which tile shape a GEMM launch uses; every variant is one template
+// instantiation in qs_gemm.hip.
+enum TileId : int {
+  TILE_128x128 = 0,   // 4 waves, 64x64 per wave (4x4 MFMA 16x16 blocks)
+  TILE_64x64 = 1,     // 4 waves, 32x32 per wave (2x2 blocks)
+  TILE_128x64 = 2,    // 4 waves, 64x32 per wave
+  TILE_64x128 = 3,    // 4 waves, 32x64 per wave
+  TILE_32x64 = 4,     // 4 waves, 16x32 per wave (small M)
+  TILE_COUNT = 5
+};
+struct TileShape { int bm, bn, bk; };
+TileShape tile_shape(int tile, int compute_type);
+
+// Everything one GEMM launch reads.  Offsets are in elements of each
+// storage; the six tables are the full-extent fill_offsets! of each map.
+struct GemmArgs {
+  const void *A;
+  const void *B;
+  void *C;
+  int64_t a_base, b_base, c_base;
+  const int64_t *mA, *kA, *nB, *kB, *mC, *nC;
+  int64_t M, N, K;
+  int a_type, b_type, c_type, compute_type;
+  int conj_a, conj_b;
+  double alpha_re, alpha_im, beta_re, beta_im;
+  int beta_case;          // 0: beta == 0, 1: beta == 1, 2: general
+  int k_splits;           // > 1: partial sums into `partials`, reduced in a fixed order
+  int64_t k_chunk;        // K steps per split (multiple of the tile's BK)
+  void *partials;         // [k_splits][planes][N][M] of the real compute type
+  int tile;               // TileId
+  int a_kfast, b_kfast;   // gather lanes along K (1) or along M / N (0)
+  int mixed;              // some storage type differs from the compute type
+};
+
+// Launch the tiled MFMA GEMM (and the fixed-order split-K reduction when
+// k_splits > 1).  Shapes and tables must already be validated.
+hipError_t launch_gemm(const GemmArgs &g, hipStream_t stream);
+
+// C = beta * C over the M x N rectangle (alpha == 0 or K == 0).
+hipError_t launch_scale(const GemmArgs &g, hipStream_t stream);
+
+// Bandwidth-bound paths.
+hipError_t launch_dot(const GemmArgs &g, hipStream_t stream);
+hipError_t launch_outer(const GemmArgs &g, hipStream_t stream);
+
+}  // namespace qs

@@ -1,0 +1,43 @@
+// The plan object behind the opaque `qs_plan*` (counterpart of ContractPlan,
+// src/planning/plan.jl:16-45, and its ContractWorkspace, workspace.jl:24-52).
+#pragma once
+
+#include "qs_internal.h"
+
+struct qs_plan {
+  qs_contract_desc desc;      // the caller's descriptor (storages may be rebound)
+  int64_t M = 0, N = 0, K = 0;
+  qs_group mg, ng, kg;        // groups in the device enumeration order
+  int path = QS_PATH_EMPTY;
+  int tile = 0;
+  int k_splits = 1;
+  int64_t k_chunk = 0;
+  int a_kfast = 0, b_kfast = 0;
+  int mixed = 0;
+  int slices = 0;
+  int64_t workspace_bytes = 0;
+  // Device offset tables: mA mC nB nC kA kB, one allocation.
+  int64_t *d_tables = nullptr;
+  const int64_t *t_mA = nullptr, *t_mC = nullptr, *t_nB = nullptr, *t_nC = nullptr, *t_kA = nullptr, *t_kB = nullptr;
+  void *d_partials = nullptr;
+  // Map ranges (exact min / max offsets of each map).
+  int64_t mA_lo = 0, mA_hi = 0, mC_lo = 0, mC_hi = 0, nB_lo = 0, nB_hi = 0, nC_lo = 0, nC_hi = 0;
+  int64_t kA_lo = 0, kA_hi = 0, kB_lo = 0, kB_hi = 0;
+  // Absolute element spans of the last bounds check.
+  int64_t a_lo = 0, a_hi = -1, b_lo = 0, b_hi = -1, c_lo = 0, c_hi = -1;
+  // Host-memory staging buffers (A, B, C spans) and their capacities in bytes.
+  void *stage[3] = {nullptr, nullptr, nullptr};
+  size_t stage_cap[3] = {0, 0, 0};
+};
+
+namespace qs {
+int64_t group_length(const qs_group &g);
+qs_status validate_group(const qs_group &g, const char *name);
+void fill_offsets(const qs_group &g, int64_t first, int64_t count, int64_t *o0, int64_t *o1);
+void map_range(const qs_group &g, int p, int64_t &lo, int64_t &hi);
+qs_status run_device(qs_plan *p, const void *A, int64_t a_base, const void *B, int64_t b_base, void *C, int64_t c_base,
+                     const double *al, const double *be, hipStream_t s);
+qs_status run_staged(qs_plan *p, const double *al, const double *be, bool reads_ab, hipStream_t s);
+GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *B, int64_t b_base, void *C,
+                   int64_t c_base);
+}  // namespace qs

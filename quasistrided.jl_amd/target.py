@@ -1,0 +1,25 @@
+"""The gfx950 TargetProfile (the device counterpart of src/hardware/target.jl)."""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _lib
+from .errors import raise_status
+
+# The MI355X datasheet figures the roofline is priced against (dense rates).
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
+PEAK_INT8_TOPS = 5000.0
+PEAK_HBM_GBS = 8000.0
+
+
+def target_profile() -> dict:
+    """hipGetDeviceProperties of the current device plus the MFMA table."""
+    p = _lib.QSTargetProfile()
+    st = _lib.lib().qs_target_profile_get(C.byref(p))
+    if st != 0:
+        raise_status(st, _lib.last_error())
+    out = {f: getattr(p, f) for f, _ in _lib.QSTargetProfile._fields_}
+    out["name"] = p.name.decode()
+    return out
