@@ -33,13 +33,19 @@ template <> struct Acc4<double> { typedef qs_d4 type; };
 template <> struct Acc4<float> { typedef qs_f4 type; };
 
 // v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32.  Lane l holds A[l%16][l/16]
-// and B[l/16][l%16]; D element r of lane l is D[4*(l/16)+r][l%16].
+// and B[l/16][l%16].  D register r of lane l is D[4*(l/16)+r][l%16] for FP32
+// but D[l/16+4r][l%16] for FP64 (checked on gfx950 with one-hot operands).
 __device__ __forceinline__ qs_d4 mfma16(double a, double b, qs_d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 __device__ __forceinline__ qs_f4 mfma16(float a, float b, qs_f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+
+// Row of D held in register r by a lane of group fk = lane/16.
+template <typename R> __device__ __forceinline__ int mfma_row(int fk, int r);
+template <> __device__ __forceinline__ int mfma_row<double>(int fk, int r) { return fk + 4 * r; }
+template <> __device__ __forceinline__ int mfma_row<float>(int fk, int r) { return 4 * fk + r; }
 
 __device__ __forceinline__ double qs_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float qs_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -293,7 +299,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs g) {
   for (int a = 0; a < FM; ++a)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int64_t row = m0 + wm * WTM + a * 16 + 4 * fk + r;
+      const int64_t row = m0 + wm * WTM + a * 16 + mfma_row<R>(fk, r);
       if (row >= g.M) continue;
 #pragma unroll
       for (int b = 0; b < FN; ++b) {
