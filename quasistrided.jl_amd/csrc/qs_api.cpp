@@ -129,16 +129,21 @@ static int unit_dim(const qs_group &g, int p) {
 
 // ------------------------------------------------------------------ tiles
 
-static int pick_bk_host(int bm, int bn, int ct) {
+// Mirrors pick_bk_for in qs_gemm_impl.h.
+static int pick_bk_host(int bm, int bn, int ct, bool akf, bool bkf) {
   const int planes = is_complex(ct) ? 2 : 1;
   const int rsize = is_double(ct) ? 8 : 4;
-  const int bk = (48 * 1024) / (planes * (bm + bn + 32) * rsize * 2);
-  int p = 4;
-  while (p * 2 <= bk && p * 2 <= 32) p *= 2;
-  return p;
+  auto rowmajor = [&](int bk) { return rsize == 8 ? bk + 2 : ((bk / 4) % 2 ? bk : bk + 4); };
+  int best = 4;
+  for (int bk = 4; bk <= 32; bk *= 2) {
+    const int a = akf ? bm * rowmajor(bk) : bk * (bm + 16);
+    const int b = bkf ? bn * rowmajor(bk) : bk * (bn + 16);
+    if (2 * planes * (a + b) * rsize <= 76 * 1024) best = bk;
+  }
+  return best;
 }
 
-TileShape tile_shape(int tile, int ct) {
+TileShape tile_shape(int tile, int ct, bool akf, bool bkf) {
   int bm = 64, bn = 64;
   switch (tile) {
     case TILE_128x128: bm = 128; bn = 128; break;
@@ -147,7 +152,7 @@ TileShape tile_shape(int tile, int ct) {
     case TILE_32x64: bm = 32; bn = 64; break;
     default: break;
   }
-  return TileShape{bm, bn, pick_bk_host(bm, bn, ct)};
+  return TileShape{bm, bn, pick_bk_host(bm, bn, ct, akf, bkf)};
 }
 
 static int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -301,7 +306,7 @@ qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
   p->k_chunk = K;
   if (p->path == QS_PATH_GEMM || p->path == QS_PATH_SKINNY) {
     p->tile = (d.tile >= 0 && d.tile < TILE_COUNT) ? d.tile : choose_tile(M, N, ct);
-    const TileShape ts = tile_shape(p->tile, ct);
+    const TileShape ts = tile_shape(p->tile, ct, p->a_kfast, p->b_kfast);
     const int64_t tiles = cdiv(M, ts.bm) * cdiv(N, ts.bn);
     if (tiles > INT32_MAX) { qs_plan_destroy(p); return fail(QS_ERR_UNSUPPORTED, "plan_contract: too many tiles"); }
     p->k_splits = choose_splits(tiles, K, ts.bk, p->k_chunk);
@@ -465,7 +470,7 @@ qs_status qs_plan_get_info(const qs_plan *p, qs_plan_info *out) {
   if (!p || !out) return fail(QS_ERR_ARGUMENT, "qs_plan_get_info: null argument");
   std::memset(out, 0, sizeof *out);
   out->path = p->path;
-  const TileShape ts = tile_shape(p->tile, p->desc.compute_type);
+  const TileShape ts = tile_shape(p->tile, p->desc.compute_type, p->a_kfast, p->b_kfast);
   out->tile_m = ts.bm;
   out->tile_n = ts.bn;
   out->tile_k = ts.bk;

@@ -50,45 +50,57 @@ template <> __device__ __forceinline__ int mfma_row<float>(int fk, int r) { retu
 __device__ __forceinline__ double qs_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float qs_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// Operand storages live in device global memory; addressing them through
+// address-space-1 pointers keeps every access a global_load/global_store
+// (a generic pointer would turn them into flat accesses, which also count
+// against the LDS wait counter).
+#define QS_GLOBAL __attribute__((address_space(1)))
+
 // Element `idx` of storage `p` as the compute type.  MIXED: the storage type
 // `t` is a runtime value (uniform across the workgroup); otherwise it is the
 // compute type itself.
 template <typename R, bool CPLX, bool MIXED>
-__device__ __forceinline__ void load_elem(const void *__restrict__ p, int t, int64_t idx, R &re, R &im) {
+__device__ __forceinline__ void load_elem(const void *p, int t, int64_t idx, R &re, R &im) {
   if (!MIXED) {
+    const QS_GLOBAL R *q = (const QS_GLOBAL R *)p;
     if (!CPLX) {
-      re = ((const R *)p)[idx];
+      re = q[idx];
       im = 0;
     } else {
-      re = ((const R *)p)[2 * idx];
-      im = ((const R *)p)[2 * idx + 1];
+      re = q[2 * idx];
+      im = q[2 * idx + 1];
     }
     return;
   }
+  const QS_GLOBAL float *f = (const QS_GLOBAL float *)p;
+  const QS_GLOBAL double *d = (const QS_GLOBAL double *)p;
   switch (t) {
-    case QS_F32: re = (R)((const float *)p)[idx]; im = 0; break;
-    case QS_F64: re = (R)((const double *)p)[idx]; im = 0; break;
-    case QS_C32: re = (R)((const float *)p)[2 * idx]; im = (R)((const float *)p)[2 * idx + 1]; break;
-    default: re = (R)((const double *)p)[2 * idx]; im = (R)((const double *)p)[2 * idx + 1]; break;
+    case QS_F32: re = (R)f[idx]; im = 0; break;
+    case QS_F64: re = (R)d[idx]; im = 0; break;
+    case QS_C32: re = (R)f[2 * idx]; im = (R)f[2 * idx + 1]; break;
+    default: re = (R)d[2 * idx]; im = (R)d[2 * idx + 1]; break;
   }
 }
 
 template <typename R, bool CPLX, bool MIXED>
-__device__ __forceinline__ void store_elem(void *__restrict__ p, int t, int64_t idx, R re, R im) {
+__device__ __forceinline__ void store_elem(void *p, int t, int64_t idx, R re, R im) {
   if (!MIXED) {
+    QS_GLOBAL R *q = (QS_GLOBAL R *)p;
     if (!CPLX) {
-      ((R *)p)[idx] = re;
+      q[idx] = re;
     } else {
-      ((R *)p)[2 * idx] = re;
-      ((R *)p)[2 * idx + 1] = im;
+      q[2 * idx] = re;
+      q[2 * idx + 1] = im;
     }
     return;
   }
+  QS_GLOBAL float *f = (QS_GLOBAL float *)p;
+  QS_GLOBAL double *d = (QS_GLOBAL double *)p;
   switch (t) {
-    case QS_F32: ((float *)p)[idx] = (float)re; break;
-    case QS_F64: ((double *)p)[idx] = (double)re; break;
-    case QS_C32: ((float *)p)[2 * idx] = (float)re; ((float *)p)[2 * idx + 1] = (float)im; break;
-    default: ((double *)p)[2 * idx] = (double)re; ((double *)p)[2 * idx + 1] = (double)im; break;
+    case QS_F32: f[idx] = (float)re; break;
+    case QS_F64: d[idx] = (double)re; break;
+    case QS_C32: f[2 * idx] = (float)re; f[2 * idx + 1] = (float)im; break;
+    default: d[2 * idx] = (double)re; d[2 * idx + 1] = (double)im; break;
   }
 }
 
@@ -131,15 +143,34 @@ __device__ __forceinline__ void store_c(const GemmArgs &g, int64_t row, int64_t 
 }
 
 // K depth of one LDS stage: the largest power of two in [4, 32] whose double-
-// buffered A and B tiles fit 48 KiB.
-template <typename R, bool CPLX, int BM, int BN>
-constexpr int pick_bk() {
-  int planes = CPLX ? 2 : 1;
-  int bk = (48 * 1024) / (planes * (BM + BN + 32) * (int)sizeof(R) * 2);
-  int p = 4;
-  while (p * 2 <= bk && p * 2 <= 32) p *= 2;
-  return p;
+// buffered A and B tiles fit the LDS budget (76 KiB: two workgroups per CU).
+constexpr int lds_pitch_rowmajor(int rsize, int bk) { return rsize == 8 ? bk + 2 : ((bk / 4) % 2 ? bk : bk + 4); }
+constexpr int pick_bk_for(int rsize, int planes, int bm, int bn, bool akf, bool bkf) {
+  int best = 4;
+  for (int bk = 4; bk <= 32; bk *= 2) {
+    int a = akf ? bm * lds_pitch_rowmajor(rsize, bk) : bk * (bm + 16);
+    int b = bkf ? bn * lds_pitch_rowmajor(rsize, bk) : bk * (bn + 16);
+    if (2 * planes * (a + b) * rsize <= 76 * 1024) best = bk;
+  }
+  return best;
 }
+template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF>
+constexpr int pick_bk() { return pick_bk_for((int)sizeof(R), CPLX ? 2 : 1, BM, BN, AKF, BKF); }
+
+// LDS geometry of one operand tile (ROWS x BK, P planes, double-buffered).
+// A gather laid along the free group (KF == false) stores the tile K-major,
+// [k][row] with a row pitch of ROWS + 16; one laid along K (KF == true)
+// stores it row-major, [row][k], with the smallest pitch >= BK that keeps a
+// fragment read free of bank conflicts (a quarter-wave's 16 rows land on
+// distinct banks).
+template <typename R, int ROWS, int BK, bool KF>
+struct LdsTile {
+  static constexpr int kpitch = ROWS + 16;
+  static constexpr int rpitch = lds_pitch_rowmajor((int)sizeof(R), BK);
+  static constexpr int plane = KF ? ROWS * rpitch : BK * kpitch;
+  __device__ static __forceinline__ int at(int row, int k) { return KF ? row * rpitch + k : k * kpitch + row; }
+};
+
 
 // XCD-aware workgroup order: the hardware deals workgroups round-robin over
 // the 8 XCDs, so consecutive ids land on different L2s.  Renumber so each XCD
@@ -149,174 +180,222 @@ __device__ __forceinline__ int64_t xcd_swizzle(int64_t bid, int64_t nwg) {
   return (bid % 8) * (nwg / 8) + bid / 8;
 }
 
+typedef const __attribute__((address_space(4))) int64_t *qs_const_i64;
+
+// Sixteen zero bytes: what a gather lane outside the tile reads.
+__device__ const double qs_zero16[2] = {0.0, 0.0};
+
+// One operand's share of the gather.  Thread t loads NL elements of the
+// ROWS x BK stage.  Laid along the free group (KF == false), its row is fixed
+// (t % ROWS) and its K steps are kk0 + r*(256/ROWS) -- uniform over a wave
+// when ROWS >= 64, so their offsets come through the scalar cache.  Laid
+// along K (KF == true), its K step is fixed (t % BK) and its rows are
+// t/BK + r*(256/BK).  Row offsets are read once; K offsets are fetched one
+// stage ahead of the data that uses them.  Loads are unconditional at
+// clamped in-range coordinates; out-of-tile values become zero.
+template <typename R, bool CPLX, bool MIXED, int ROWS, int BK, bool KF>
+struct Gather {
+  static constexpr int P = CPLX ? 2 : 1;
+  static constexpr int NL = (ROWS * BK + 255) / 256;
+  static constexpr int NR = KF ? NL : 1;           // distinct rows per thread
+  static constexpr int NK = KF ? 1 : NL;           // distinct K steps per thread
+  static constexpr int RSTEP = KF ? 256 / BK : 0;  // row step between loads
+  static constexpr int KSTEP = KF ? 0 : 256 / ROWS;
+  static constexpr bool UNIFORM_K = !KF && ROWS >= 64;
+  static constexpr bool PARTIAL = ROWS * BK % 256 != 0;
+  int64_t rowoff[NR];
+  bool rowok[NR];
+  int row0, kk0;
+  int64_t koff[NK];
+  bool kok[NK];
+  R val[NL][P];
+
+  __device__ __forceinline__ void init(int t, int64_t r0, int64_t rows_total, int64_t base,
+                                       const int64_t *__restrict__ rowtab) {
+    row0 = KF ? t / BK : t % ROWS;
+    kk0 = KF ? t % BK : t / ROWS;
+    if (UNIFORM_K) kk0 = __builtin_amdgcn_readfirstlane(kk0);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int64_t gi = r0 + row0 + r * RSTEP;
+      rowok[r] = gi < rows_total && (!PARTIAL || t + 256 * r < ROWS * BK);
+      rowoff[r] = base + rowtab[gi < rows_total ? gi : rows_total - 1];
+    }
+  }
+  __device__ __forceinline__ void fetch_k(int64_t k0, int64_t kend, const int64_t *__restrict__ ktab) {
+#pragma unroll
+    for (int r = 0; r < NK; ++r) {
+      const int64_t gk = k0 + kk0 + r * KSTEP;
+      kok[r] = gk < kend;
+      const int64_t gkc = gk < kend ? gk : kend - 1;
+      koff[r] = UNIFORM_K ? ((qs_const_i64)ktab)[gkc] : ktab[gkc];
+    }
+  }
+  // Issue the stage's loads.  An element outside the tile reads the device
+  // zero instead (a select on the address, not on the value), so nothing
+  // waits on the loads until `stage` consumes them.
+  __device__ __forceinline__ void load(const void *__restrict__ p, int type) {
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      const int ir = KF ? r : 0, kr = KF ? 0 : r;
+      const bool ok = rowok[ir] && kok[kr];
+      const void *src = ok ? p : (const void *)qs_zero16;
+      const int64_t idx = ok ? rowoff[ir] + koff[kr] : 0;
+      R im;
+      load_elem<R, CPLX, MIXED>(src, type, idx, val[r][0], im);
+      if (CPLX) val[r][P - 1] = im;
+    }
+  }
+  template <int PLANE>
+  __device__ __forceinline__ void stage(R *lds, int t, R sgn) {
+    typedef LdsTile<R, ROWS, BK, KF> L;
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      if (!PARTIAL || t + 256 * r < ROWS * BK) {
+        const int row = row0 + (KF ? r * RSTEP : 0);
+        const int k = kk0 + (KF ? 0 : r * KSTEP);
+        lds[L::at(row, k)] = val[r][0];
+        if (CPLX) lds[PLANE + L::at(row, k)] = sgn * val[r][P - 1];
+      }
+    }
+  }
+};
+
+// One BM x BN tile of C over the K range of split `split`: the whole
+// workgroup's work, shared by the one-contraction kernel and the grouped
+// persistent kernel.  `smem` holds SMEM elements of R.
 template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED>
-__global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs g) {
-  constexpr int P = CPLX ? 2 : 1;
-  constexpr int BK = pick_bk<R, CPLX, BM, BN>();
-  constexpr int PAD = 16;
-  constexpr int LDA = BM + PAD, LDB = BN + PAD;
-  constexpr int WTM = BM / 2, WTN = BN / 2;       // 2 x 2 waves
-  constexpr int FM = WTM / 16, FN = WTN / 16;     // MFMA blocks per wave
-  constexpr int NLA = (BM * BK + 255) / 256, NLB = (BN * BK + 255) / 256;
-  static_assert(256 % BM == 0 || BM % 256 == 0, "BM");
+struct TileGemm {
+  static constexpr int P = CPLX ? 2 : 1;
+  static constexpr int BK = pick_bk<R, CPLX, BM, BN, AKF, BKF>();
+  static constexpr int WTM = BM / 2, WTN = BN / 2;       // 2 x 2 waves
+  static constexpr int FM = WTM / 16, FN = WTN / 16;     // MFMA blocks per wave
+  typedef LdsTile<R, BM, BK, AKF> LA;
+  typedef LdsTile<R, BN, BK, BKF> LB;
+  static constexpr int SA = P * LA::plane, SB = P * LB::plane;   // one stage of A / B
+  static constexpr int SMEM = 2 * (SA + SB);
+  static_assert(256 % BM == 0 && 256 % BN == 0, "tile rows");
   static_assert(BK % 4 == 0, "BK");
-  typedef typename Acc4<R>::type V;
 
-  __shared__ R As[2][P][BK][LDA];
-  __shared__ R Bs[2][P][BK][LDB];
+  __device__ static void run(const GemmArgs &g, int64_t tm, int64_t tn, int split, R *smem) {
+    typedef typename Acc4<R>::type V;
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int64_t m0 = tm * BM, n0 = tn * BN;
+    const int64_t kbeg = (int64_t)split * g.k_chunk;
+    const int64_t kend = min(g.K, kbeg + g.k_chunk);
 
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+    Gather<R, CPLX, MIXED, BM, BK, AKF> ga;
+    Gather<R, CPLX, MIXED, BN, BK, BKF> gb;
+    ga.init(t, m0, g.M, g.a_base, g.mA);
+    gb.init(t, n0, g.N, g.b_base, g.nB);
+    const R sa = g.conj_a ? (R)-1 : (R)1, sb = g.conj_b ? (R)-1 : (R)1;
 
+    V acc[P][FM][FN];
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) acc[p][a][b] = V{0, 0, 0, 0};
+
+    const int fr = lane & 15, fk = lane >> 4;
+    int buf = 0;
+    if (kbeg < kend) {
+      ga.fetch_k(kbeg, kend, g.kA);
+      gb.fetch_k(kbeg, kend, g.kB);
+      ga.load(g.A, g.a_type);
+      gb.load(g.B, g.b_type);
+      if (kbeg + BK < kend) {
+        ga.fetch_k(kbeg + BK, kend, g.kA);
+        gb.fetch_k(kbeg + BK, kend, g.kB);
+      }
+      ga.template stage<LA::plane>(smem, t, sa);
+      gb.template stage<LB::plane>(smem + SA, t, sb);
+    }
+    __syncthreads();
+    for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+      const bool more = k0 + BK < kend;
+      if (more) {
+        ga.load(g.A, g.a_type);
+        gb.load(g.B, g.b_type);
+        if (k0 + 2 * BK < kend) {
+          ga.fetch_k(k0 + 2 * BK, kend, g.kA);
+          gb.fetch_k(k0 + 2 * BK, kend, g.kB);
+        }
+      }
+      const R *As = smem + buf * (SA + SB);
+      const R *Bs = As + SA;
+#pragma unroll
+      for (int ks = 0; ks < BK / 4; ++ks) {
+        R af[P][FM], bf[P][FN];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+#pragma unroll
+          for (int a = 0; a < FM; ++a) af[p][a] = As[p * LA::plane + LA::at(wm * WTM + a * 16 + fr, ks * 4 + fk)];
+#pragma unroll
+          for (int b = 0; b < FN; ++b) bf[p][b] = Bs[p * LB::plane + LB::at(wn * WTN + b * 16 + fr, ks * 4 + fk)];
+        }
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) {
+            if (!CPLX) {
+              acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+            } else {
+              acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+              acc[0][a][b] = mfma16(-af[P - 1][a], bf[P - 1][b], acc[0][a][b]);
+              acc[P - 1][a][b] = mfma16(af[0][a], bf[P - 1][b], acc[P - 1][a][b]);
+              acc[P - 1][a][b] = mfma16(af[P - 1][a], bf[0][b], acc[P - 1][a][b]);
+            }
+          }
+      }
+      if (more) {
+        R *Ws = smem + (buf ^ 1) * (SA + SB);
+        ga.template stage<LA::plane>(Ws, t, sa);
+        gb.template stage<LB::plane>(Ws + SA, t, sb);
+      }
+      __syncthreads();
+      buf ^= 1;
+    }
+
+    // Epilogue.
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm * WTM + a * 16 + mfma_row<R>(fk, r);
+        if (row >= g.M) continue;
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          const int64_t col = n0 + wn * WTN + b * 16 + fr;
+          if (col >= g.N) continue;
+          const R xr = acc[0][a][b][r];
+          const R xi = CPLX ? acc[P - 1][a][b][r] : (R)0;
+          if (g.k_splits == 1) {
+            store_c<R, CPLX, MIXED>(g, row, col, xr, xi);
+          } else {
+            R *part = (R *)g.partials;
+            const int64_t plane = g.M * g.N;
+            part[((int64_t)split * P) * plane + row * g.N + col] = xr;
+            if (CPLX) part[((int64_t)split * P + 1) * plane + row * g.N + col] = xi;
+          }
+        }
+      }
+  }
+};
+
+template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED>
+__global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmArgs g) {
+  typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, MIXED> TG;
+  __shared__ R smem[TG::SMEM];
   const int64_t tiles_m = (g.M + BM - 1) / BM;
   const int64_t tiles_n = (g.N + BN - 1) / BN;
   const int64_t bid = xcd_swizzle(blockIdx.x, (int64_t)gridDim.x);
   const int64_t tm = bid % tiles_m, tn = bid / tiles_m;
   if (tn >= tiles_n) return;
-  const int64_t m0 = tm * BM, n0 = tn * BN;
-  const int split = blockIdx.y;
-  const int64_t kbeg = (int64_t)split * g.k_chunk;
-  const int64_t kend = min(g.K, kbeg + g.k_chunk);
-
-  // Per-thread gather coordinates, fixed over the K loop.
-  int64_t a_row[NLA];
-  int a_kk[NLA];
-  bool a_ok[NLA];
-#pragma unroll
-  for (int r = 0; r < NLA; ++r) {
-    const int e = t + 256 * r;
-    const int i = AKF ? e / BK : e % BM;
-    a_kk[r] = AKF ? e % BK : e / BM;
-    const int64_t gi = m0 + i;
-    a_ok[r] = (e < BM * BK) && gi < g.M;
-    a_row[r] = a_ok[r] ? g.a_base + g.mA[gi] : 0;
-  }
-  int64_t b_col[NLB];
-  int b_kk[NLB];
-  bool b_ok[NLB];
-#pragma unroll
-  for (int r = 0; r < NLB; ++r) {
-    const int e = t + 256 * r;
-    const int j = BKF ? e / BK : e % BN;
-    b_kk[r] = BKF ? e % BK : e / BN;
-    const int64_t gj = n0 + j;
-    b_ok[r] = (e < BN * BK) && gj < g.N;
-    b_col[r] = b_ok[r] ? g.b_base + g.nB[gj] : 0;
-  }
-
-  R ra[NLA][P], rb[NLB][P];
-  const R sa = g.conj_a ? (R)-1 : (R)1, sb = g.conj_b ? (R)-1 : (R)1;
-
-  auto gather = [&](int64_t k0) {
-#pragma unroll
-    for (int r = 0; r < NLA; ++r) {
-      const int64_t gk = k0 + a_kk[r];
-      R re = 0, im = 0;
-      if (a_ok[r] && gk < kend) load_elem<R, CPLX, MIXED>(g.A, g.a_type, a_row[r] + g.kA[gk], re, im);
-      ra[r][0] = re;
-      if (CPLX) ra[r][P - 1] = sa * im;
-    }
-#pragma unroll
-    for (int r = 0; r < NLB; ++r) {
-      const int64_t gk = k0 + b_kk[r];
-      R re = 0, im = 0;
-      if (b_ok[r] && gk < kend) load_elem<R, CPLX, MIXED>(g.B, g.b_type, b_col[r] + g.kB[gk], re, im);
-      rb[r][0] = re;
-      if (CPLX) rb[r][P - 1] = sb * im;
-    }
-  };
-  auto stage = [&](int buf) {
-#pragma unroll
-    for (int r = 0; r < NLA; ++r) {
-      const int e = t + 256 * r;
-      if (e < BM * BK) {
-        const int i = AKF ? e / BK : e % BM;
-#pragma unroll
-        for (int p = 0; p < P; ++p) As[buf][p][a_kk[r]][i] = ra[r][p];
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < NLB; ++r) {
-      const int e = t + 256 * r;
-      if (e < BN * BK) {
-        const int j = BKF ? e / BK : e % BN;
-#pragma unroll
-        for (int p = 0; p < P; ++p) Bs[buf][p][b_kk[r]][j] = rb[r][p];
-      }
-    }
-  };
-
-  V acc[P][FM][FN];
-#pragma unroll
-  for (int p = 0; p < P; ++p)
-#pragma unroll
-    for (int a = 0; a < FM; ++a)
-#pragma unroll
-      for (int b = 0; b < FN; ++b) acc[p][a][b] = V{0, 0, 0, 0};
-
-  const int fr = lane & 15, fk = lane >> 4;
-  int buf = 0;
-  if (kbeg < kend) {
-    gather(kbeg);
-    stage(0);
-  }
-  __syncthreads();
-  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
-    const bool more = k0 + BK < kend;
-    if (more) gather(k0 + BK);
-#pragma unroll
-    for (int ks = 0; ks < BK / 4; ++ks) {
-      R af[P][FM], bf[P][FN];
-#pragma unroll
-      for (int p = 0; p < P; ++p) {
-#pragma unroll
-        for (int a = 0; a < FM; ++a) af[p][a] = As[buf][p][ks * 4 + fk][wm * WTM + a * 16 + fr];
-#pragma unroll
-        for (int b = 0; b < FN; ++b) bf[p][b] = Bs[buf][p][ks * 4 + fk][wn * WTN + b * 16 + fr];
-      }
-#pragma unroll
-      for (int a = 0; a < FM; ++a)
-#pragma unroll
-        for (int b = 0; b < FN; ++b) {
-          if (!CPLX) {
-            acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
-          } else {
-            acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
-            acc[0][a][b] = mfma16(-af[P - 1][a], bf[P - 1][b], acc[0][a][b]);
-            acc[P - 1][a][b] = mfma16(af[0][a], bf[P - 1][b], acc[P - 1][a][b]);
-            acc[P - 1][a][b] = mfma16(af[P - 1][a], bf[0][b], acc[P - 1][a][b]);
-          }
-        }
-    }
-    if (more) stage(buf ^ 1);
-    __syncthreads();
-    buf ^= 1;
-  }
-
-  // Epilogue.
-#pragma unroll
-  for (int a = 0; a < FM; ++a)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int64_t row = m0 + wm * WTM + a * 16 + mfma_row<R>(fk, r);
-      if (row >= g.M) continue;
-#pragma unroll
-      for (int b = 0; b < FN; ++b) {
-        const int64_t col = n0 + wn * WTN + b * 16 + fr;
-        if (col >= g.N) continue;
-        const R xr = acc[0][a][b][r];
-        const R xi = CPLX ? acc[P - 1][a][b][r] : (R)0;
-        if (g.k_splits == 1) {
-          store_c<R, CPLX, MIXED>(g, row, col, xr, xi);
-        } else {
-          R *part = (R *)g.partials;
-          const int64_t plane = g.M * g.N;
-          part[((int64_t)split * P) * plane + row * g.N + col] = xr;
-          if (CPLX) part[((int64_t)split * P + 1) * plane + row * g.N + col] = xi;
-        }
-      }
-    }
+  TG::run(g, tm, tn, blockIdx.y, smem);
 }
 
 // Split-K reduction: partial sums added in split order 0, 1, ..., S-1 (a

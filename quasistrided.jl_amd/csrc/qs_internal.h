@@ -30,7 +30,7 @@ enum TileId : int {
   TILE_COUNT = 5
 };
 struct TileShape { int bm, bn, bk; };
-TileShape tile_shape(int tile, int compute_type);
+TileShape tile_shape(int tile, int compute_type, bool a_kfast, bool b_kfast);
 
 // Everything one GEMM launch reads.  Offsets are in elements of each
 // storage; the six tables are the full-extent fill_offsets! of each map.
