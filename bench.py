@@ -96,6 +96,79 @@ def config1(qs, torch, dev):
     return plan, 2 * 2304 * 96 * 96, 8 * (2304 * 96 + 96 * 96 + 2304 * 96), (A, B, C), 0.0
 
 
+def config3(qs, torch, dev, cplx, n=512, seed=3):
+    """512 MPS environment contractions L[a,w,a'] * psi[a',s,b] -> C[a,w,s,b],
+    w = 5, s uniform in {2,3,4}, a, a', b log-uniform in [16, 512]."""
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    dt = torch.complex128 if cplx else torch.float64
+    SV = qs.StridedView
+    plans, keep, flops = [], [], 0
+    for _ in range(n):
+        a, ap, b = (int(round(np.exp(rng.uniform(np.log(16), np.log(512))))) for _ in range(3))
+        s = int(rng.integers(2, 5))
+        w = 5
+        L = torch.randn(a * w * ap, dtype=dt, device=dev, generator=g)
+        P = torch.randn(ap * s * b, dtype=dt, device=dev, generator=g)
+        Cst = torch.zeros(a * w * s * b, dtype=dt, device=dev)
+        Lv = SV(L, (a, w, ap), (1, a, a * w))
+        Pv = SV(P, (ap, s, b), (1, ap, ap * s))
+        Cv = SV(Cst, (a, w, s, b), (1, a, a * w, a * w * s))
+        plans.append(qs.plan_contract(Cv, Lv, (1, 2, 3), Pv, (3, 4, 5), (1, 2, 4, 5)))
+        keep.append((L, P, Cst))
+        flops += (8 if cplx else 2) * a * w * ap * s * b
+    return plans, keep, flops
+
+
+def bench_grouped(qs, torch, dev, cplx, reps=5):
+    """Config-3 sweep: per-launch plain execution of the 512 plans vs one
+    grouped persistent launch vs its captured HIP graph."""
+    plans, keep, flops = config3(qs, torch, dev, cplx)
+    n = len(plans)
+    for p in plans:
+        qs.execute(p, 1.0, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for p in plans:
+            qs.execute(p, 1.0, 0.0)
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / reps
+    grp = qs.GroupPlan(plans)
+    grp.execute(1.0, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        grp.execute(1.0, 0.0)
+    torch.cuda.synchronize()
+    t_group = (time.perf_counter() - t0) / reps
+    graph = grp.capture(repeats=1, alphas=1.0, betas=0.0)
+    graph.launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        graph.launch()
+    e1.record()
+    torch.cuda.synchronize()
+    t_graph = e0.elapsed_time(e1) * 1e-3 / reps
+    out = {
+        "contractions": n,
+        "gflop_per_sweep": round(flops / 1e9, 2),
+        "per_launch_contractions_per_s": round(n / t_plain, 1),
+        "grouped_contractions_per_s": round(n / t_group, 1),
+        "graph_contractions_per_s": round(n / t_graph, 1),
+        "graph_vs_per_launch": round(t_plain / t_graph, 2),
+        "graph_tflops": round(flops / t_graph / 1e12, 2),
+        "graph_mfma_frac": round(flops / t_graph / PEAK_FP64, 4),
+    }
+    graph.release()
+    grp.release()
+    del plans, keep
+    return out
+
+
 def time_plan(qs, torch, plan, beta, steps, warmup):
     """Warm up, then time `steps` executes back to back (wall clock between
     synchronizations) and each execute with HIP events on the launch stream."""
@@ -161,6 +234,7 @@ def main():
 
     plan, flops, bytes_, views, beta = config2(qs, torch, np.float64, dev)
     c_init = views[3].parent.cpu().numpy().copy()
+    tune = qs.autotune(plan, reps=10)
     wall, kern_mean, kern_med = time_plan(qs, torch, plan, beta, args.steps, args.warmup)
     value = flops * args.steps / wall / 1e9
     achieved = flops / kern_mean
@@ -182,6 +256,7 @@ def main():
             "workload": "C[a,b,c,d] := A[a,e,b,f]*conj(B[f,c,e,d]), all extents 32 (M=N=K=1024), Float64, "
                         "A a stride-2 slice of a 64-wide parent on b, beta=0.5, C prefilled",
             "plan": repr(plan),
+            "autotune": tune,
             "flops_per_step": flops,
         },
         "roofline": {
@@ -219,11 +294,14 @@ def main():
                             ("c_anb_48x96_f64", lambda: config1(qs, torch, dev)),
                             ("permuted_all32_c64", lambda: config2(qs, torch, np.complex128, dev))):
             p, fl, by, vw, be = maker()
+            qs.autotune(p, reps=3)
             steps = 20 if name.startswith("rank4") else 100
             w, km, _ = time_plan(qs, torch, p, be, steps, 3)
             extra[name] = {"gflops": round(fl * steps / w / 1e9, 1), "kernel_ms": round(km * 1e3, 4),
                            "mfma_frac": round(fl / km / PEAK_FP64, 4), "plan": repr(p)}
             del p, vw
+        extra["grouped_512_mps_f64"] = bench_grouped(qs, torch, dev, False)
+        extra["grouped_512_mps_c64"] = bench_grouped(qs, torch, dev, True)
         result["extra"] = extra
     print(json.dumps(result), flush=True)
 

@@ -94,6 +94,8 @@ typedef struct qs_contract_desc {
   double target_rel_err;            /* split precision: error to meet (0: default) */
   int32_t tile;                     /* tile-config override, -1 = automatic  */
   int32_t path_mode;                /* 0 auto; 1 force the tiled MFMA GEMM path */
+  int32_t k_splits;                 /* split-K override, 0 = automatic       */
+  int32_t _reserved;
 } qs_contract_desc;
 
 typedef struct qs_plan qs_plan;
@@ -176,6 +178,65 @@ qs_status qs_plan_rebind(qs_plan *plan, const void *a, int64_t a_base, int64_t a
 /* Free a plan and its device tables.  Counterpart of `release!`
  * (src/execution/workspace.jl:110-123).  NULL is a no-op. */
 qs_status qs_plan_destroy(qs_plan *plan);
+
+/* ---- autotuner ----------------------------------------------------------
+ * No counterpart in the reference (whose blocking is an analytic cache model,
+ * src/planning/blocking.jl:66-77): the GPU tile and split-K factor are chosen
+ * by timing candidates.  Results are kept per shape class (compute and
+ * storage types, gather directions, half-octave buckets of M, N, K) in a
+ * process-wide cache that `qs_plan_create` consults, and that can be saved
+ * to and loaded from a text file. */
+typedef struct qs_tune_result {
+  int32_t tile, k_splits;        /* chosen configuration            */
+  double best_ms;                /* its mean time per execute       */
+  double default_ms;             /* the untuned plan's time         */
+  int32_t candidates;            /* configurations timed            */
+  int32_t _pad;
+} qs_tune_result;
+
+/* Time every candidate (tile, split-K) on the plan's own operands (device
+ * plans only; results go to a scratch copy of C's span, so C is not
+ * modified), adopt the fastest for this plan and record it for its shape
+ * class. `reps` timed executes per candidate (0 = 5). */
+qs_status qs_autotune(qs_plan *plan, void *stream, int32_t reps, qs_tune_result *out);
+
+/* Save / load / clear the shape-class cache (text, one entry per line). */
+qs_status qs_autotune_save(const char *path);
+qs_status qs_autotune_load(const char *path);
+qs_status qs_autotune_clear(void);
+
+/* ---- grouped execution ----------------------------------------------------
+ * No counterpart in the reference, which runs one contraction per call
+ * (src/execution/execute.jl:11-20): tensor-network sweeps issue many small
+ * contractions of uneven shape, and a group runs all of them in one
+ * persistent launch per compute type and gather direction.  Members are
+ * existing device-memory plans on the tiled GEMM path (they must outlive the
+ * group; rebinding a member's storages requires rebuilding the group).
+ * Each member computes all of K in registers, so its result does not depend
+ * on the other members. */
+typedef struct qs_group_plan qs_group_plan;
+typedef struct qs_graph qs_graph;
+
+/* `tiles`: per-member workgroup tile (qs TileId 0..4, -1 = the grouped
+ * heuristic), or NULL for all heuristic. */
+qs_status qs_group_create(qs_plan *const *plans, int32_t n, const int32_t *tiles, qs_group_plan **out);
+
+/* Run every member: C_i = alpha_i A_i B_i + beta_i C_i.  alphas / betas hold
+ * two doubles per member (re, im; im ignored for real members); NULL means
+ * alpha = 1 / beta = 0.  Spans of every member are checked before launch. */
+qs_status qs_group_execute(qs_group_plan *g, const double *alphas, const double *betas, void *stream);
+
+/* Algorithmic flops of one execute (2 per real MAC, 8 per complex MAC). */
+qs_status qs_group_flops(const qs_group_plan *g, int64_t *flops);
+
+qs_status qs_group_destroy(qs_group_plan *g);
+
+/* Capture `repeats` back-to-back executes of the group (with the given
+ * coefficients, uploaded before capture) into a HIP graph. */
+qs_status qs_group_capture(qs_group_plan *g, const double *alphas, const double *betas, int32_t repeats,
+                           void *stream, qs_graph **out);
+qs_status qs_graph_launch(qs_graph *gr, void *stream);
+qs_status qs_graph_destroy(qs_graph *gr);
 
 /* Host-side offset enumeration, exactly `fill_offsets!`
  * (src/layout/axis_group.jl:87-145) for a two-map group: writes `count`

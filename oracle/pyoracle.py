@@ -37,6 +37,7 @@ class _Desc(C.Structure):
         ("b", C.c_void_p), ("b_base", C.c_int64), ("b_len", C.c_int64),
         ("c", C.c_void_p), ("c_base", C.c_int64), ("c_len", C.c_int64),
         ("target_rel_err", C.c_double), ("tile", C.c_int32), ("path_mode", C.c_int32),
+        ("k_splits", C.c_int32), ("_reserved", C.c_int32),
     ]
 
 

@@ -45,6 +45,7 @@ class QSContractDesc(C.Structure):
         ("target_rel_err", C.c_double),
         ("tile", C.c_int32),
         ("path_mode", C.c_int32),
+        ("k_splits", C.c_int32), ("_reserved", C.c_int32),
     ]
 
 
@@ -54,6 +55,11 @@ class QSPlanInfo(C.Structure):
         ("k_splits", C.c_int32), ("a_kfast", C.c_int32), ("b_kfast", C.c_int32), ("slices", C.c_int32),
         ("m", C.c_int64), ("n", C.c_int64), ("k", C.c_int64), ("workspace_bytes", C.c_int64),
     ]
+
+
+class QSTuneResult(C.Structure):
+    _fields_ = [("tile", C.c_int32), ("k_splits", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double),
+                ("candidates", C.c_int32), ("_pad", C.c_int32)]
 
 
 class QSTargetProfile(C.Structure):
@@ -82,6 +88,18 @@ SIGNATURES = {
     "qs_plan_rebind": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
                                  C.c_void_p, C.c_int64, C.c_int64]),
     "qs_plan_destroy": (C.c_int, [C.c_void_p]),
+    "qs_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(QSTuneResult)]),
+    "qs_autotune_save": (C.c_int, [C.c_char_p]),
+    "qs_autotune_load": (C.c_int, [C.c_char_p]),
+    "qs_autotune_clear": (C.c_int, []),
+    "qs_group_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_void_p)]),
+    "qs_group_execute": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]),
+    "qs_group_flops": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "qs_group_destroy": (C.c_int, [C.c_void_p]),
+    "qs_group_capture": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_void_p,
+                                   C.POINTER(C.c_void_p)]),
+    "qs_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "qs_graph_destroy": (C.c_int, [C.c_void_p]),
     "qs_fill_offsets": (C.c_int, [C.POINTER(QSGroup), C.c_int64, C.c_int64, C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64)]),
 }

@@ -300,31 +300,58 @@ qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
   p->t_kA = p->t_nC + N;
   p->t_kB = p->t_kA + K;
 
-  // Tile, splits, workspace.
+  // Tile, splits, workspace: an explicit override, else the autotune cache's
+  // choice for this shape class, else the heuristic.
   p->tile = TILE_64x64;
   p->k_splits = 1;
   p->k_chunk = K;
   if (p->path == QS_PATH_GEMM || p->path == QS_PATH_SKINNY) {
-    p->tile = (d.tile >= 0 && d.tile < TILE_COUNT) ? d.tile : choose_tile(M, N, ct);
-    const TileShape ts = tile_shape(p->tile, ct, p->a_kfast, p->b_kfast);
-    const int64_t tiles = cdiv(M, ts.bm) * cdiv(N, ts.bn);
-    if (tiles > INT32_MAX) { qs_plan_destroy(p); return fail(QS_ERR_UNSUPPORTED, "plan_contract: too many tiles"); }
-    p->k_splits = choose_splits(tiles, K, ts.bk, p->k_chunk);
-    if (p->k_splits > 1) {
-      const size_t bytes = (size_t)p->k_splits * (is_complex(ct) ? 2 : 1) * (size_t)M * (size_t)N * bytes_of_compute_real(ct);
-      e = hipMalloc(&p->d_partials, bytes);
-      if (e != hipSuccess) {
-        // No room for partial sums: run unsplit.
-        p->d_partials = nullptr;
-        p->k_splits = 1;
-        p->k_chunk = K;
-        (void)hipGetLastError();
-      } else {
-        p->workspace_bytes = (int64_t)bytes;
-      }
+    int tile = (d.tile >= 0 && d.tile < TILE_COUNT) ? d.tile : -1;
+    int splits = d.k_splits > 0 ? d.k_splits : 0;
+    if (tile < 0 && splits == 0) {
+      int t2, s2;
+      if (tune_lookup(p, t2, s2)) { tile = t2; splits = s2; }
     }
+    if (tile < 0) tile = choose_tile(M, N, ct);
+    qs_status cs = configure_gemm(p, tile, splits);
+    if (cs != QS_OK) { qs_plan_destroy(p); return cs; }
   }
   *out = p;
+  return QS_OK;
+}
+
+qs_status configure_gemm(qs_plan *p, int tile, int splits) {
+  const int ct = p->desc.compute_type;
+  const TileShape ts = tile_shape(tile, ct, p->a_kfast, p->b_kfast);
+  const int64_t tiles = cdiv(p->M, ts.bm) * cdiv(p->N, ts.bn);
+  if (tiles > INT32_MAX) return fail(QS_ERR_UNSUPPORTED, "plan_contract: too many tiles");
+  int ks;
+  int64_t chunk;
+  if (splits <= 0) {
+    ks = choose_splits(tiles, p->K, ts.bk, chunk);
+  } else {
+    chunk = cdiv(cdiv(p->K, splits), ts.bk) * ts.bk;
+    ks = (int)cdiv(p->K, chunk);
+  }
+  const size_t need = ks > 1 ? (size_t)ks * (is_complex(ct) ? 2 : 1) * (size_t)p->M * (size_t)p->N *
+                                   bytes_of_compute_real(ct)
+                             : 0;
+  if (need > (size_t)p->workspace_bytes) {
+    if (p->d_partials) (void)hipFree(p->d_partials);
+    p->d_partials = nullptr;
+    p->workspace_bytes = 0;
+    if (hipMalloc(&p->d_partials, need) != hipSuccess) {
+      (void)hipGetLastError();
+      p->d_partials = nullptr;
+      ks = 1;   // no room for partial sums: run unsplit
+      chunk = p->K;
+    } else {
+      p->workspace_bytes = (int64_t)need;
+    }
+  }
+  p->tile = tile;
+  p->k_splits = ks;
+  p->k_chunk = chunk;
   return QS_OK;
 }
 

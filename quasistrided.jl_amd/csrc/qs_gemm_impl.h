@@ -400,18 +400,22 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmArgs g) {
 
 // Split-K reduction: partial sums added in split order 0, 1, ..., S-1 (a
 // fixed order, so results do not depend on scheduling), then the epilogue.
+// Columns across threads (the partials are row-major, so loads coalesce),
+// rows across blockIdx.y with a grid-stride loop.
 template <typename R, bool CPLX, bool MIXED>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(const GemmArgs g) {
   constexpr int P = CPLX ? 2 : 1;
   const int64_t plane = g.M * g.N;
   const R *part = (const R *)g.partials;
-  for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < plane; e += (int64_t)gridDim.x * 256) {
+  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (col >= g.N) return;
+  for (int64_t row = blockIdx.y; row < g.M; row += gridDim.y) {
+    const int64_t e = row * g.N + col;
     R xr = part[e], xi = CPLX ? part[plane + e] : (R)0;
     for (int s = 1; s < g.k_splits; ++s) {
       xr += part[((int64_t)s * P) * plane + e];
       if (CPLX) xi += part[((int64_t)s * P + 1) * plane + e];
     }
-    const int64_t row = e / g.N, col = e % g.N;
     store_c<R, CPLX, MIXED>(g, row, col, xr, xi);
   }
 }
@@ -430,10 +434,9 @@ static hipError_t launch_tile_mixed(const GemmArgs &g, hipStream_t s) {
     hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, false, false, MIXED>), grid, block, 0, s, g);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess || g.k_splits == 1) return err;
-  const int64_t n = g.M * g.N;
-  int64_t blocks = (n + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL((splitk_reduce_kernel<R, CPLX, MIXED>), dim3((unsigned)blocks), dim3(256), 0, s, g);
+  const int64_t bx = (g.N + 255) / 256;
+  const int64_t by = g.M < 65535 ? g.M : 65535;
+  hipLaunchKernelGGL((splitk_reduce_kernel<R, CPLX, MIXED>), dim3((unsigned)bx, (unsigned)by), dim3(256), 0, s, g);
   return hipGetLastError();
 }
 

@@ -1,0 +1,382 @@
+// Grouped persistent execution: many differently shaped and strided
+// contractions in one launch.
+//
+// The tensor-network sweeps this engine serves (MPS/DMRG environments, bond
+// dimensions from 16 to 512) issue many small pairwise contractions; one
+// launch per contraction leaves the GPU idle between them.  A group is built
+// from existing plans: their GEMM descriptors (offset tables, storages, tile
+// class) go into one device table; a persistent grid walks the union of their
+// tiles, each workgroup taking tiles t, t + grid, t + 2 grid, ... and finding
+// the owning contraction by binary search over the tile prefix sums.  Every
+// tile computes all of K in registers (no split-K), so a contraction's result
+// does not depend on what else is in the group.  Contractions whose compute
+// type or gather directions differ go to separate launches (one kernel
+// instantiation each) on the same stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "qs_gemm_impl.h"
+#include "qs_plan.h"
+
+namespace qs {
+
+struct GroupItem {
+  GemmArgs g;           // alpha / beta are taken from the coefficient table
+  int64_t tile_start;   // first global tile index of this contraction
+  int64_t tiles_m;
+  int32_t cls;          // TileId of its workgroup tile
+  int32_t index;        // position in the caller's plan array
+};
+
+// The tile classes a grouped launch holds: 128x64, 64x64 and 32x64 (the
+// three bodies share one register allocation, and 128x128 alone needs ~230
+// VGPRs).  Their largest LDS footprint:
+template <typename R, bool CPLX, bool AKF, bool BKF>
+struct GroupSmem {
+  static constexpr int a = TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::SMEM;
+  static constexpr int c = TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::SMEM;
+  static constexpr int d = TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::SMEM;
+  static constexpr int cd = c > d ? c : d;
+  static constexpr int value = a > cd ? a : cd;
+};
+
+// coef: per item {alpha_re, alpha_im, beta_re, beta_im}, indexed by item.index.
+template <typename R, bool CPLX, bool AKF, bool BKF>
+__global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__restrict__ items, int n_items,
+                                                          int64_t total_tiles, const double *__restrict__ coef) {
+  __shared__ R smem[GroupSmem<R, CPLX, AKF, BKF>::value];
+  for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+    int lo = 0, hi = n_items - 1;
+    while (lo < hi) {   // last item with tile_start <= t
+      const int mid = (lo + hi + 1) >> 1;
+      if (items[mid].tile_start <= t) lo = mid; else hi = mid - 1;
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    GemmArgs g = items[lo].g;
+    const int64_t local = t - items[lo].tile_start;
+    const int64_t tm = local % items[lo].tiles_m, tn = local / items[lo].tiles_m;
+    const double *c = coef + 4 * items[lo].index;
+    g.alpha_re = c[0];
+    g.alpha_im = c[1];
+    g.beta_re = c[2];
+    g.beta_im = c[3];
+    g.beta_case = (c[2] == 0 && c[3] == 0) ? 0 : (c[2] == 1 && c[3] == 0) ? 1 : 2;
+    switch (items[lo].cls) {
+      case TILE_128x64: TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::run(g, tm, tn, 0, smem); break;
+      case TILE_32x64: TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::run(g, tm, tn, 0, smem); break;
+      default: TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::run(g, tm, tn, 0, smem); break;
+    }
+  }
+}
+
+template <typename R, bool CPLX, bool AKF, bool BKF>
+static hipError_t launch_grouped_t(const GroupItem *items, int n, int64_t total, const double *coef, int grid,
+                                   hipStream_t s) {
+  hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF>), dim3(grid), dim3(256), 0, s, items, n, total, coef);
+  return hipGetLastError();
+}
+
+template <typename R, bool CPLX>
+static hipError_t launch_grouped_rc(int akf, int bkf, const GroupItem *items, int n, int64_t total,
+                                    const double *coef, int grid, hipStream_t s) {
+  switch ((akf ? 2 : 0) | (bkf ? 1 : 0)) {
+    case 0: return launch_grouped_t<R, CPLX, false, false>(items, n, total, coef, grid, s);
+    case 1: return launch_grouped_t<R, CPLX, false, true>(items, n, total, coef, grid, s);
+    case 2: return launch_grouped_t<R, CPLX, true, false>(items, n, total, coef, grid, s);
+    default: return launch_grouped_t<R, CPLX, true, true>(items, n, total, coef, grid, s);
+  }
+}
+
+static hipError_t launch_grouped(int ct, int akf, int bkf, const GroupItem *items, int n, int64_t total,
+                                 const double *coef, int grid, hipStream_t s) {
+  switch (ct) {
+    case QS_F64: return launch_grouped_rc<double, false>(akf, bkf, items, n, total, coef, grid, s);
+    case QS_F32: return launch_grouped_rc<float, false>(akf, bkf, items, n, total, coef, grid, s);
+    case QS_C64: return launch_grouped_rc<double, true>(akf, bkf, items, n, total, coef, grid, s);
+    default: return launch_grouped_rc<float, true>(akf, bkf, items, n, total, coef, grid, s);
+  }
+}
+
+// Workgroup tile of one contraction inside a group: the grid is already
+// full of other contractions' tiles, so take the tile that wastes the least
+// padded work, weighted by how much slower the smaller register tiles run.
+int grouped_tile(int64_t M, int64_t N, int ct) {
+  struct Cand { int id, bm, bn; double cost; };
+  const Cand cands[] = {{TILE_128x64, 128, 64, 1.0}, {TILE_64x64, 64, 64, 1.2}, {TILE_32x64, 32, 64, 1.7}};
+  int best = TILE_64x64;
+  double best_cost = 1e300;
+  for (const Cand &c : cands) {
+    const double padded = (double)((M + c.bm - 1) / c.bm * c.bm) * (double)((N + c.bn - 1) / c.bn * c.bn);
+    const double cost = padded * c.cost;
+    if (cost < best_cost) { best_cost = cost; best = c.id; }
+  }
+  return best;
+}
+
+struct Launch {   // one kernel instantiation's share of the group
+  int ct, akf, bkf, mixed;
+  GroupItem *d_items = nullptr;
+  int n = 0;
+  int64_t total = 0;
+  int grid = 0;
+};
+
+}  // namespace qs
+
+struct qs_group_plan {
+  std::vector<qs_plan *> plans;
+  std::vector<int> tile_override;       // per plan, -1 = heuristic
+  std::vector<qs::Launch> launches;
+  std::vector<qs_plan *> standalone;    // members with mixed storage types
+  std::vector<int> standalone_index;
+  double *d_coef = nullptr;             // 4 doubles per plan
+  double *h_coef = nullptr;             // pinned host staging
+  std::vector<double> last_coef;
+  int64_t flops = 0;
+};
+
+struct qs_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace qs {
+
+static int cu_count() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipDeviceProp_t pr;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess) cus = pr.multiProcessorCount;
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
+
+static void free_launches(qs_group_plan *g) {
+  for (auto &l : g->launches)
+    if (l.d_items) (void)hipFree(l.d_items);
+  g->launches.clear();
+  g->standalone.clear();
+}
+
+// (Re)build the device tables from the plans and their tile classes.
+static qs_status build_tables(qs_group_plan *g) {
+  free_launches(g);
+  std::map<std::tuple<int, int, int, int>, std::vector<GroupItem>> classes;
+  for (size_t i = 0; i < g->plans.size(); ++i) {
+    qs_plan *p = g->plans[i];
+    GemmArgs a = make_args(p, p->desc.a, p->desc.a_base, p->desc.b, p->desc.b_base, p->desc.c, p->desc.c_base);
+    a.k_splits = 1;
+    a.k_chunk = p->K;
+    a.partials = nullptr;
+    GroupItem it;
+    std::memset(&it, 0, sizeof it);
+    it.g = a;
+    if (p->mixed) {   // mixed storage types run standalone, on the same stream
+      g->standalone.push_back(p);
+      continue;
+    }
+    it.cls = g->tile_override[i] >= 0 ? g->tile_override[i] : grouped_tile(p->M, p->N, p->desc.compute_type);
+    if (it.cls != TILE_128x64 && it.cls != TILE_32x64) it.cls = TILE_64x64;
+    const TileShape ts = tile_shape(it.cls, p->desc.compute_type, p->a_kfast, p->b_kfast);
+    it.tiles_m = (p->M + ts.bm - 1) / ts.bm;
+    it.tile_start = it.tiles_m * ((p->N + ts.bn - 1) / ts.bn);   // tile count, prefix-summed below
+    it.index = (int32_t)i;
+    classes[std::make_tuple(p->desc.compute_type, p->a_kfast, p->b_kfast, 0)].push_back(it);
+  }
+  for (auto &kv : classes) {
+    std::vector<GroupItem> &v = kv.second;
+    // Longest K first: a tile's cost grows with K, so the persistent grid's
+    // round-robin hands out the long tiles first.
+    std::stable_sort(v.begin(), v.end(), [](const GroupItem &x, const GroupItem &y) { return x.g.K > y.g.K; });
+    int64_t start = 0;
+    for (auto &it : v) {
+      const int64_t n = it.tile_start;
+      it.tile_start = start;
+      start += n;
+    }
+    Launch l;
+    l.ct = std::get<0>(kv.first);
+    l.akf = std::get<1>(kv.first);
+    l.bkf = std::get<2>(kv.first);
+    l.mixed = std::get<3>(kv.first);
+    l.n = (int)v.size();
+    l.total = start;
+    l.grid = (int)std::min<int64_t>(start, 2 * (int64_t)cu_count());
+    if (l.grid < 1) l.grid = 1;
+    if (hipMalloc(&l.d_items, sizeof(GroupItem) * v.size()) != hipSuccess) {
+      free_launches(g);
+      return fail(QS_ERR_MEMORY, "qs_group_create: device table");
+    }
+    if (hipMemcpy(l.d_items, v.data(), sizeof(GroupItem) * v.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(l.d_items);
+      free_launches(g);
+      return fail(QS_ERR_DEVICE, "qs_group_create: table upload");
+    }
+    g->launches.push_back(l);
+  }
+  return QS_OK;
+}
+
+static qs_status set_coefficients(qs_group_plan *g, const double *alphas, const double *betas, hipStream_t s) {
+  const size_t n = g->plans.size();
+  std::vector<double> c(4 * n);
+  for (size_t i = 0; i < n; ++i) {
+    c[4 * i] = alphas ? alphas[2 * i] : 1.0;
+    c[4 * i + 1] = alphas ? alphas[2 * i + 1] : 0.0;
+    c[4 * i + 2] = betas ? betas[2 * i] : 0.0;
+    c[4 * i + 3] = betas ? betas[2 * i + 1] : 0.0;
+  }
+  if (c == g->last_coef) return QS_OK;
+  // The previous copy out of the staging buffer may still be in flight.
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(QS_ERR_DEVICE, "group: coefficient sync");
+  std::memcpy(g->h_coef, c.data(), sizeof(double) * c.size());
+  if (hipMemcpyAsync(g->d_coef, g->h_coef, sizeof(double) * c.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail(QS_ERR_DEVICE, "group: coefficient upload");
+  g->last_coef.swap(c);
+  return QS_OK;
+}
+
+static qs_status launch_all(qs_group_plan *g, hipStream_t s) {
+  for (const Launch &l : g->launches) {
+    if (l.total == 0) continue;
+    hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, l.d_items, l.n, l.total, g->d_coef, l.grid, s);
+    if (e != hipSuccess) return fail(QS_ERR_DEVICE, std::string("group: launch: ") + hipGetErrorString(e));
+  }
+  for (qs_plan *p : g->standalone) {
+    size_t i = 0;
+    while (g->plans[i] != p) ++i;
+    const double *c = g->last_coef.data() + 4 * i;
+    qs_status st = run_device(p, p->desc.a, p->desc.a_base, p->desc.b, p->desc.b_base, p->desc.c, p->desc.c_base,
+                              c, c + 2, s);
+    if (st != QS_OK) return st;
+  }
+  return QS_OK;
+}
+
+}  // namespace qs
+
+using namespace qs;
+
+extern "C" {
+
+qs_status qs_group_create(qs_plan *const *plans, int32_t n, const int32_t *tiles, qs_group_plan **out) {
+  if (!out) return fail(QS_ERR_ARGUMENT, "qs_group_create: null output");
+  *out = nullptr;
+  if (n < 0 || (n > 0 && !plans)) return fail(QS_ERR_ARGUMENT, "qs_group_create: bad plan array");
+  for (int32_t i = 0; i < n; ++i) {
+    const qs_plan *p = plans[i];
+    if (!p) return fail(QS_ERR_ARGUMENT, "qs_group_create: null plan " + std::to_string(i));
+    if (p->desc.memory != QS_MEM_DEVICE)
+      return fail(QS_ERR_ARGUMENT, "qs_group_create: plan " + std::to_string(i) + " is not in device memory");
+    if (p->path != QS_PATH_GEMM && p->path != QS_PATH_SKINNY)
+      return fail(QS_ERR_ARGUMENT, "qs_group_create: plan " + std::to_string(i) +
+                                       " does not run the tiled GEMM path (degenerate extents)");
+    if (tiles && (tiles[i] < -1 || tiles[i] >= TILE_COUNT))
+      return fail(QS_ERR_ARGUMENT, "qs_group_create: bad tile for plan " + std::to_string(i));
+  }
+  qs_group_plan *g = new qs_group_plan();
+  g->plans.assign(plans, plans + n);
+  g->tile_override.assign(n, -1);
+  if (tiles) g->tile_override.assign(tiles, tiles + n);
+  for (qs_plan *p : g->plans) g->flops += (is_complex(p->desc.compute_type) ? 8 : 2) * p->M * p->N * p->K;
+  const size_t cb = sizeof(double) * 4 * (size_t)std::max<int32_t>(n, 1);
+  if (hipMalloc(&g->d_coef, cb) != hipSuccess || hipHostMalloc(&g->h_coef, cb, hipHostMallocDefault) != hipSuccess) {
+    qs_group_destroy(g);
+    return fail(QS_ERR_MEMORY, "qs_group_create: coefficient buffers");
+  }
+  qs_status st = build_tables(g);
+  if (st != QS_OK) {
+    qs_group_destroy(g);
+    return st;
+  }
+  *out = g;
+  return QS_OK;
+}
+
+qs_status qs_group_execute(qs_group_plan *g, const double *alphas, const double *betas, void *stream) {
+  if (!g) return fail(QS_ERR_ARGUMENT, "qs_group_execute: null group");
+  hipStream_t s = (hipStream_t)stream;
+  // Exact span checks for every member before anything is launched.
+  for (qs_plan *p : g->plans) {
+    const qs_contract_desc &d = p->desc;
+    const int64_t clo = d.c_base + p->mC_lo + p->nC_lo, chi = d.c_base + p->mC_hi + p->nC_hi;
+    const int64_t alo = d.a_base + p->mA_lo + p->kA_lo, ahi = d.a_base + p->mA_hi + p->kA_hi;
+    const int64_t blo = d.b_base + p->kB_lo + p->nB_lo, bhi = d.b_base + p->kB_hi + p->nB_hi;
+    if (clo < 0 || chi >= d.c_len || alo < 0 || ahi >= d.a_len || blo < 0 || bhi >= d.b_len)
+      return fail(QS_ERR_BOUNDS, "qs_group_execute: a member's operand span exceeds its storage");
+  }
+  qs_status st = set_coefficients(g, alphas, betas, s);
+  if (st != QS_OK) return st;
+  return launch_all(g, s);
+}
+
+qs_status qs_group_flops(const qs_group_plan *g, int64_t *flops) {
+  if (!g || !flops) return fail(QS_ERR_ARGUMENT, "qs_group_flops: null argument");
+  *flops = g->flops;
+  return QS_OK;
+}
+
+qs_status qs_group_destroy(qs_group_plan *g) {
+  if (!g) return QS_OK;
+  free_launches(g);
+  if (g->d_coef) (void)hipFree(g->d_coef);
+  if (g->h_coef) (void)hipHostFree(g->h_coef);
+  delete g;
+  return QS_OK;
+}
+
+qs_status qs_group_capture(qs_group_plan *g, const double *alphas, const double *betas, int32_t repeats, void *stream,
+                           qs_graph **out) {
+  if (!g || !out) return fail(QS_ERR_ARGUMENT, "qs_group_capture: null argument");
+  *out = nullptr;
+  if (repeats < 1) return fail(QS_ERR_ARGUMENT, "qs_group_capture: repeats must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  // Coefficients are uploaded before capture: the graph holds kernel nodes only.
+  qs_status st = set_coefficients(g, alphas, betas, s);
+  if (st != QS_OK) return st;
+  hipStream_t cs;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+    return fail(QS_ERR_DEVICE, "qs_group_capture: stream");
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    return fail(QS_ERR_DEVICE, "qs_group_capture: sync");
+  }
+  qs_graph *gr = new qs_graph();
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  for (int r = 0; r < repeats && e == hipSuccess && st == QS_OK; ++r) st = launch_all(g, cs);
+  hipError_t e2 = hipStreamEndCapture(cs, &gr->graph);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess && st == QS_OK) e = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
+  (void)hipStreamDestroy(cs);
+  if (e != hipSuccess || st != QS_OK) {
+    qs_graph_destroy(gr);
+    return st != QS_OK ? st : fail(QS_ERR_DEVICE, std::string("qs_group_capture: ") + hipGetErrorString(e));
+  }
+  *out = gr;
+  return QS_OK;
+}
+
+qs_status qs_graph_launch(qs_graph *gr, void *stream) {
+  if (!gr || !gr->exec) return fail(QS_ERR_ARGUMENT, "qs_graph_launch: null graph");
+  hipError_t e = hipGraphLaunch(gr->exec, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(QS_ERR_DEVICE, std::string("qs_graph_launch: ") + hipGetErrorString(e));
+  return QS_OK;
+}
+
+qs_status qs_graph_destroy(qs_graph *gr) {
+  if (!gr) return QS_OK;
+  if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+  if (gr->graph) (void)hipGraphDestroy(gr->graph);
+  delete gr;
+  return QS_OK;
+}
+
+}  // extern "C"

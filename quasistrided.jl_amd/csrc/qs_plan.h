@@ -38,6 +38,12 @@ void map_range(const qs_group &g, int p, int64_t &lo, int64_t &hi);
 qs_status run_device(qs_plan *p, const void *A, int64_t a_base, const void *B, int64_t b_base, void *C, int64_t c_base,
                      const double *al, const double *be, hipStream_t s);
 qs_status run_staged(qs_plan *p, const double *al, const double *be, bool reads_ab, hipStream_t s);
+// Set a GEMM plan's tile and split-K factor (0 splits: the heuristic),
+// reallocating the partial-sum workspace as needed.
+qs_status configure_gemm(qs_plan *p, int tile, int splits);
+// The autotune cache's choice for the plan's shape class, if any.
+bool tune_lookup(const qs_plan *p, int &tile, int &splits);
+int grouped_tile(int64_t M, int64_t N, int ct);
 GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *B, int64_t b_base, void *C,
                    int64_t c_base);
 }  // namespace qs

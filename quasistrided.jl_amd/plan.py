@@ -44,6 +44,7 @@ class MFMAKernel:
     of the reference), and an optional workgroup-tile override (TileId)."""
     m_tile: int = DEFAULT_M_TILE
     tile: int = -1
+    k_splits: int = 0
 
     def tile_size(self, i: int = 1) -> int:
         return self.m_tile if i == 1 else 16
@@ -152,6 +153,43 @@ class ContractPlan:
                 f"k_splits={self.info['k_splits']}, swapped={self.swapped})")
 
 
+def _refresh_info(plan: ContractPlan):
+    info = _lib.QSPlanInfo()
+    st = _lib.lib().qs_plan_get_info(plan.handle, C.byref(info))
+    if st != 0:
+        raise_status(st, _lib.last_error())
+    plan.info = {f: getattr(info, f) for f, _ in _lib.QSPlanInfo._fields_}
+
+
+def autotune(plan: ContractPlan, reps: int = 5, stream=None) -> dict:
+    """Time every (workgroup tile, split-K) candidate on the plan's own
+    operands (C is not modified), adopt the fastest, and remember it for the
+    plan's shape class; later plans of that class start from it."""
+    r = _lib.QSTuneResult()
+    s = C.c_void_p(stream.cuda_stream) if stream is not None else _stream_of(plan)
+    st = _lib.lib().qs_autotune(plan.handle, s, int(reps), C.byref(r))
+    if st != 0:
+        raise_status(st, _lib.last_error())
+    _refresh_info(plan)
+    return {f: getattr(r, f) for f, _ in _lib.QSTuneResult._fields_ if not f.startswith("_")}
+
+
+def autotune_save(path: str):
+    st = _lib.lib().qs_autotune_save(str(path).encode())
+    if st != 0:
+        raise_status(st, _lib.last_error())
+
+
+def autotune_load(path: str):
+    st = _lib.lib().qs_autotune_load(str(path).encode())
+    if st != 0:
+        raise_status(st, _lib.last_error())
+
+
+def autotune_clear():
+    _lib.lib().qs_autotune_clear()
+
+
 def release(plan: ContractPlan):
     """`release!` (workspace.jl:118-123): free the plan's device memory."""
     plan.release()
@@ -226,6 +264,7 @@ def planned(C_, A_, indA, B_, indB, indC, *, kernel=None, conjA=False, conjB=Fal
     d.c, d.c_base, d.c_len = Cv.parent.data_ptr(), Cv.offset, Cv.parent.numel()
     d.target_rel_err = float(target_error)
     d.tile = int(getattr(kern, "tile", -1))
+    d.k_splits = int(getattr(kern, "k_splits", 0))
     d.path_mode = 1 if (modes.dot == "never" and modes.outer == "never") else 0
     handle = C.c_void_p()
     st = _lib.lib().qs_plan_create(C.byref(d), C.byref(handle))

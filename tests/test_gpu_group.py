@@ -1,0 +1,116 @@
+"""Grouped persistent execution and graph capture on the GPU: every member
+against the oracle, the same bits alone and in the mix, graph replays equal
+to eager runs."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import RTOL, SV, oracle_of_plan, qs, rel_fro, storage_np
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def mps_env(rng, dtype, a, ap, b, sigma, w=5):
+    """L[a,w,a'] * psi[a',s,b] -> C[a,w,s,b] (column-major, device)."""
+    def rnd(n):
+        if np.dtype(dtype).kind == "c":
+            return (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(dtype)
+        return rng.standard_normal(n).astype(dtype)
+    L = SV(torch.from_numpy(rnd(a * w * ap)).to(DEV), (a, w, ap), (1, a, a * w))
+    P = SV(torch.from_numpy(rnd(ap * sigma * b)).to(DEV), (ap, sigma, b), (1, ap, ap * sigma))
+    Cv = SV(torch.from_numpy(rnd(a * w * sigma * b)).to(DEV), (a, w, sigma, b), (1, a, a * w, a * w * sigma))
+    return Cv, L, P
+
+
+def dims(rng, n, lo=16, hi=200):
+    out = []
+    for _ in range(n):
+        a, ap, b = (int(round(np.exp(rng.uniform(np.log(lo), np.log(hi))))) for _ in range(3))
+        out.append((a, ap, b, int(rng.integers(2, 5))))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32])
+def test_group_members_match_oracle_and_standalone(oracle, dtype):
+    case(f"group of 24 MPS env contractions {np.dtype(dtype).name}")
+    rng = np.random.default_rng(3)
+    shapes = dims(rng, 24)
+    views, plans, starts = [], [], []
+    for (a, ap, b, s) in shapes:
+        Cv, L, P = mps_env(rng, dtype, a, ap, b, s)
+        views.append((Cv, L, P))
+        starts.append(storage_np(Cv.parent))
+        plans.append(qs.plan_contract(Cv, L, (1, 2, 3), P, (3, 4, 5), (1, 2, 4, 5)))
+    cplx = np.dtype(dtype).kind == "c"
+    alphas = [(1.0 + 0.25j * i) if cplx else 1.0 + 0.1 * i for i in range(len(plans))]
+    betas = [(0.5 - 0.5j) if cplx else 0.5 for _ in plans]
+    grp = qs.GroupPlan(plans)
+    grp.execute(alphas, betas)
+    torch.cuda.synchronize()
+    mixed = [storage_np(v[0].parent) for v in views]
+    for i, (plan, (Cv, L, P)) in enumerate(zip(plans, views)):
+        want = oracle_of_plan(oracle, plan, alphas[i], betas[i], storage_np(plan.Astorage), storage_np(plan.Bstorage),
+                              starts[i])
+        assert rel_fro(mixed[i], want) <= RTOL[plan.T], (i, plan)
+    # alone vs in the mix: the same bits
+    for i in (0, 5, 17):
+        Cv = views[i][0]
+        Cv.parent.copy_(torch.from_numpy(starts[i]).to(DEV))
+        alone = qs.GroupPlan([plans[i]])
+        alone.execute([alphas[i]], [betas[i]])
+        torch.cuda.synchronize()
+        assert np.array_equal(storage_np(Cv.parent), mixed[i]), i
+        alone.release()
+    grp.release()
+
+
+def test_group_graph_replays_equal_eager(oracle):
+    case("group graph capture, 3 replays with eager work between")
+    rng = np.random.default_rng(9)
+    shapes = dims(rng, 16)
+    views, plans = [], []
+    for (a, ap, b, s) in shapes:
+        Cv, L, P = mps_env(rng, np.float64, a, ap, b, s)
+        views.append(Cv)
+        plans.append(qs.plan_contract(Cv, L, (1, 2, 3), P, (3, 4, 5), (1, 2, 4, 5)))
+    grp = qs.GroupPlan(plans)
+    grp.execute(1.0, 0.0)
+    torch.cuda.synchronize()
+    eager = [storage_np(v.parent) for v in views]
+    graph = grp.capture(repeats=2, alphas=1.0, betas=0.0)
+    for r in range(3):
+        for v in views:
+            v.parent.fill_(float("nan"))
+        graph.launch()
+        torch.cuda.synchronize()
+        x = torch.randn(1000, device=DEV).sum().item()   # eager work between replays
+        assert np.isfinite(x)
+        for i, v in enumerate(views):
+            assert np.array_equal(storage_np(v.parent), eager[i]), (r, i)
+    graph.release()
+    grp.release()
+
+
+def test_group_mixed_types_run_standalone(oracle):
+    case("group with mixed-storage members")
+    rng = np.random.default_rng(5)
+    plans, views, starts = [], [], []
+    for k in range(4):
+        M, K, N = 70 + k, 50, 40
+        A = SV(torch.from_numpy(rng.standard_normal(M * K).astype(np.float32)).to(DEV), (M, K), (1, M))
+        B = SV(torch.from_numpy(rng.standard_normal(K * N)).to(DEV), (K, N), (1, K))
+        Cv = SV(torch.from_numpy(rng.standard_normal(M * N)).to(DEV), (M, N), (1, M))
+        starts.append(storage_np(Cv.parent))
+        views.append(Cv)
+        plans.append(qs.plan_contract(Cv, A, (1, 2), B, (2, 3), (1, 3)))
+    grp = qs.GroupPlan(plans)
+    grp.execute(1.5, 0.5)
+    torch.cuda.synchronize()
+    for i, p in enumerate(plans):
+        want = oracle_of_plan(oracle, p, 1.5, 0.5, storage_np(p.Astorage), storage_np(p.Bstorage), starts[i])
+        assert rel_fro(storage_np(views[i].parent), want) <= 1e-13

@@ -420,3 +420,28 @@ def test_config4_rank4_f64_checksum():
         col = Am @ Bm[:, q]
         scale = np.abs(Am) @ np.abs(Bm[:, q])
         assert np.all(np.abs(Cm[:, q] - col) <= 1e-12 * scale + 1e-300)
+
+
+def test_autotune_keeps_results_and_c(oracle):
+    case("autotune on the permuted all-32 F64 contraction")
+    rng = np.random.default_rng(2)
+    d = 32
+    P = randn(rng, np.float64, d * d * 2 * d * d)
+    pst = colmajor_strides((d, d, 2 * d, d))
+    A = dev_view(P, (d,) * 4, (pst[0], pst[1], 2 * pst[2], pst[3]))
+    B = dev_view(randn(rng, np.float64, d ** 4), (d,) * 4, colmajor_strides((d,) * 4))
+    C = dev_view(randn(rng, np.float64, d ** 4), (d,) * 4, colmajor_strides((d,) * 4))
+    qs.autotune_clear()
+    plan = qs.plan_contract(C, A, (1, 2, 3, 4), B, (4, 5, 2, 6), (1, 3, 5, 6))
+    c0 = storage_np(C.parent)
+    res = qs.autotune(plan, reps=3)
+    torch.cuda.synchronize()
+    assert np.array_equal(storage_np(C.parent), c0)          # tuning writes a scratch C
+    assert res["candidates"] > 1 and res["best_ms"] <= res["default_ms"]
+    print("[tune]", res, plan, flush=True)
+    # the tuned plan still matches the oracle; a new plan of the class starts tuned
+    run_and_check(oracle, C, A, (1, 2, 3, 4), B, (4, 5, 2, 6), (1, 3, 5, 6), 1.0, 0.5)
+    p2 = qs.plan_contract(C, A, (1, 2, 3, 4), B, (4, 5, 2, 6), (1, 3, 5, 6))
+    assert (p2.info["tile_m"], p2.info["tile_n"], p2.info["k_splits"]) == \
+        (plan.info["tile_m"], plan.info["tile_n"], plan.info["k_splits"])
+    qs.autotune_clear()

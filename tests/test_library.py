@@ -97,3 +97,22 @@ def test_plan_create_overflow_policy():
 
 def test_destroy_null_is_noop():
     assert qs._lib.lib().qs_plan_destroy(None) == 0
+
+
+def test_autotune_cache_file_roundtrip(tmp_path):
+    qs.autotune_clear()
+    p = tmp_path / "tune.txt"
+    qs.autotune_save(str(p))
+    assert p.read_text().startswith("#")
+    p.write_text("1,1,1,1,0,1,20,20,20 1 2 0.5\n")
+    qs.autotune_load(str(p))
+    q = tmp_path / "again.txt"
+    qs.autotune_save(str(q))
+    assert "1,1,1,1,0,1,20,20,20 1 2" in q.read_text()
+    bad = tmp_path / "bad.txt"
+    bad.write_text("garbage\n")
+    with pytest.raises(qs.ArgumentError):
+        qs.autotune_load(str(bad))
+    with pytest.raises(qs.ArgumentError):
+        qs.autotune_load(str(tmp_path / "missing.txt"))
+    qs.autotune_clear()
