@@ -169,6 +169,104 @@ def bench_grouped(qs, torch, dev, cplx, reps=5):
     return out
 
 
+def served_pools(qs, torch, dev):
+    """Resident operands and plans of the served stream: small permuted
+    C[a,b,c,d] := A[a,e,b,f] * B[f,c,e,d] (all extents d in {8, 12, 16}) and
+    large C[a,n,b] := A[a,k,b] * B[k,n] with M = N = K in {1024, 2048, 3072,
+    4096}; each in Float64 and ComplexF64.  Every shape has one plan and a
+    ring of output buffers."""
+    SV = qs.StridedView
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    pools = {"small": [], "large": []}
+    for dt in (torch.float64, torch.complex128):
+        for d in (8, 12, 16):
+            A = torch.randn(d ** 4, dtype=dt, device=dev, generator=g)
+            B = torch.randn(d ** 4, dtype=dt, device=dev, generator=g)
+            Cs = [torch.zeros(d ** 4, dtype=dt, device=dev) for _ in range(64)]
+            st = colmajor((d,) * 4)
+            plan = qs.plan_contract(SV(Cs[0], (d,) * 4, st), SV(A, (d,) * 4, st), (1, 2, 3, 4), SV(B, (d,) * 4, st),
+                                    (4, 5, 2, 6), (1, 3, 5, 6))
+            fl = (8 if dt.is_complex else 2) * d ** 6
+            pools["small"].append((plan, [SV(c, (d,) * 4, st) for c in Cs], fl))
+        for n in (1024, 2048, 3072, 4096):
+            a = b = 32 if n <= 2048 else 64
+            a = int(np.sqrt(n)) if int(np.sqrt(n)) ** 2 == n else a
+            b = n // a
+            A = torch.randn(a * n * b, dtype=dt, device=dev, generator=g)
+            B = torch.randn(n * n, dtype=dt, device=dev, generator=g)
+            Cs = [torch.zeros(a * n * b, dtype=dt, device=dev) for _ in range(4)]
+            plan = qs.plan_contract(SV(Cs[0], (a, n, b), colmajor((a, n, b))), SV(A, (a, n, b), colmajor((a, n, b))),
+                                    (1, 2, 3), SV(B, (n, n), (1, n)), (2, 4), (1, 4, 3))
+            fl = (8 if dt.is_complex else 2) * n ** 3
+            pools["large"].append((plan, [SV(c, (a, n, b), colmajor((a, n, b))) for c in Cs], fl))
+    return pools
+
+
+def bench_served(qs, torch, dev, pools, grouped, rate=2000.0, duration=2.0, seed=5):
+    """Poisson arrivals at `rate` req/s for `duration` s: 90 % small, 10 %
+    large, F64 and C64; 200 GB admission budget.  Returns latency and
+    throughput figures."""
+    rng = np.random.default_rng(seed)
+    n = int(rate * duration)
+    gaps = rng.exponential(1.0 / rate, n)
+    arrivals = np.cumsum(gaps)
+    kinds = rng.random(n) < 0.9
+    srv = qs.Server(budget_bytes=200 * 10 ** 9, max_batch=128, grouped=grouped)
+    ring = {}
+    tickets, flops = [], 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lag = []
+    for i in range(n):
+        pool = pools["small"] if kinds[i] else pools["large"]
+        plan, outs, fl = pool[int(rng.integers(len(pool)))]
+        k = ring.get(id(plan), 0)
+        ring[id(plan)] = k + 1
+        target = t0 + arrivals[i]
+        while time.perf_counter() < target:
+            pass
+        lag.append(time.perf_counter() - target)
+        tickets.append(srv.submit(plan, 1.0, 0.0, Cout=outs[k % len(outs)]))
+        flops += fl
+    srv.drain()
+    st = srv.stats()
+    for t in tickets:
+        srv.poll(t)
+    srv.close()
+    return {
+        "requests": n, "offered_req_per_s": rate, "mode": "grouped" if grouped else "per-request launch",
+        "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
+        "throughput_req_per_s": round(st["throughput_rps"], 1), "tflops": round(st["tflops"], 2),
+        "mean_batch": round(st["mean_batch"], 2), "peak_gb": round(st["peak_bytes"] / 1e9, 3),
+        "submit_lag_p99_ms": round(float(np.percentile(lag, 99)) * 1e3, 3),
+    }
+
+
+def bench_saturated(qs, torch, dev, pools, grouped, n=3000, seed=6, small_share=0.9):
+    """All requests submitted at once (closed loop): the loop's capacity."""
+    rng = np.random.default_rng(seed)
+    kinds = rng.random(n) < small_share
+    srv = qs.Server(budget_bytes=200 * 10 ** 9, max_batch=256, grouped=grouped)
+    ring = {}
+    torch.cuda.synchronize()
+    tickets = []
+    for i in range(n):
+        pool = pools["small"] if kinds[i] else pools["large"]
+        plan, outs, fl = pool[int(rng.integers(len(pool)))]
+        k = ring.get(id(plan), 0)
+        ring[id(plan)] = k + 1
+        tickets.append(srv.submit(plan, 1.0, 0.0, Cout=outs[k % len(outs)]))
+    srv.drain()
+    st = srv.stats()
+    for t in tickets:
+        srv.poll(t)
+    srv.close()
+    return {"mode": "grouped" if grouped else "per-request launch", "requests": n,
+            "throughput_req_per_s": round(st["throughput_rps"], 1), "tflops": round(st["tflops"], 2),
+            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 2)}
+
+
 def time_plan(qs, torch, plan, beta, steps, warmup):
     """Warm up, then time `steps` executes back to back (wall clock between
     synchronizations) and each execute with HIP events on the launch stream."""
@@ -302,6 +400,15 @@ def main():
             del p, vw
         extra["grouped_512_mps_f64"] = bench_grouped(qs, torch, dev, False)
         extra["grouped_512_mps_c64"] = bench_grouped(qs, torch, dev, True)
+        pools = served_pools(qs, torch, dev)
+        extra["served_poisson_2k"] = {"grouped": bench_served(qs, torch, dev, pools, True),
+                                      "per_request": bench_served(qs, torch, dev, pools, False)}
+        extra["served_saturated"] = {"grouped": bench_saturated(qs, torch, dev, pools, True),
+                                     "per_request": bench_saturated(qs, torch, dev, pools, False)}
+        extra["served_saturated_small_only"] = {
+            "grouped": bench_saturated(qs, torch, dev, pools, True, n=20000, small_share=1.0),
+            "per_request": bench_saturated(qs, torch, dev, pools, False, n=20000, small_share=1.0)}
+        del pools
         result["extra"] = extra
     print(json.dumps(result), flush=True)
 

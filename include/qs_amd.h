@@ -238,6 +238,41 @@ qs_status qs_group_capture(qs_group_plan *g, const double *alphas, const double 
 qs_status qs_graph_launch(qs_graph *gr, void *stream);
 qs_status qs_graph_destroy(qs_graph *gr);
 
+/* ---- serving -------------------------------------------------------------
+ * No counterpart in the reference.  A continuous-batching request loop over
+ * operands and plans resident in HBM: requests are admitted first-come
+ * first-served while the bytes they hold (their operand spans) stay within
+ * `budget_bytes`; admitted requests are packed into grouped launches on one of
+ * four streams, so new work overlaps work already on the device.  Each request
+ * computes its K sum unsplit with the tile of its own shape, so its result is
+ * the same alone and in any mix.  `grouped == 0` runs every request as its
+ * own launch, in arrival order on one stream (the per-request baseline). */
+typedef struct qs_server qs_server;
+typedef struct qs_server_stats {
+  int64_t submitted, completed, rejected, batches;
+  double mean_batch;
+  double p50_ms, p90_ms, p99_ms, max_ms, mean_ms;   /* submit -> completion observed */
+  double span_s;            /* first submit -> last completion */
+  double throughput_rps, tflops;
+  int64_t peak_bytes, budget_bytes;
+} qs_server_stats;
+
+qs_status qs_server_create(int64_t budget_bytes, int32_t max_batch, int32_t grouped, qs_server **out);
+
+/* Submit `plan` against the given storages (NULL: the plan's own) with
+ * alpha / beta as two doubles each (re, im).  The plan must outlive the
+ * request; the storages must stay resident until it completes.  A request
+ * whose spans alone exceed the budget is rejected with QS_ERR_MEMORY. */
+qs_status qs_server_submit(qs_server *s, qs_plan *plan, const void *a, int64_t a_base, int64_t a_len,
+                           const void *b, int64_t b_base, int64_t b_len, void *c, int64_t c_base, int64_t c_len,
+                           const double *alpha, const double *beta, uint64_t *ticket);
+qs_status qs_server_poll(qs_server *s, uint64_t ticket, int32_t *done, double *latency_ms);
+qs_status qs_server_wait(qs_server *s, uint64_t ticket, double *latency_ms);
+qs_status qs_server_drain(qs_server *s);
+qs_status qs_server_stats_get(qs_server *s, qs_server_stats *out);
+qs_status qs_server_reset_stats(qs_server *s);
+qs_status qs_server_destroy(qs_server *s);
+
 /* Host-side offset enumeration, exactly `fill_offsets!`
  * (src/layout/axis_group.jl:87-145) for a two-map group: writes `count`
  * offsets of coordinates [first, first+count) into out0/out1.  Exposed so a

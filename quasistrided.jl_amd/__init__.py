@@ -16,6 +16,7 @@ from .plan import (ContractPlan, MFMAKernel, PathModes, autotune, autotune_clear
 from .backend import QuasiStridedBackend, contraction_labels, tensorcontract
 from .target import target_profile
 from .group import GroupPlan, Graph
+from .server import Server
 from . import _lib
 
 __all__ = [
@@ -24,5 +25,5 @@ __all__ = [
     "map_ramp_step", "order_contract_labels", "order_free_labels", "prefer_swap", "ContractPlan", "MFMAKernel",
     "PathModes", "autotune", "autotune_clear", "autotune_load", "autotune_save", "compute_type", "contract", "contract_", "execute", "execute_", "isconj", "op_conjugates",
     "plan_contract", "planned", "release", "QuasiStridedBackend", "contraction_labels", "tensorcontract",
-    "target_profile", "GroupPlan", "Graph",
+    "target_profile", "GroupPlan", "Graph", "Server",
 ]

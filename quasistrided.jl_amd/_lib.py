@@ -62,6 +62,14 @@ class QSTuneResult(C.Structure):
                 ("candidates", C.c_int32), ("_pad", C.c_int32)]
 
 
+class QSServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64), ("batches", C.c_int64),
+                ("mean_batch", C.c_double), ("p50_ms", C.c_double), ("p90_ms", C.c_double), ("p99_ms", C.c_double),
+                ("max_ms", C.c_double), ("mean_ms", C.c_double), ("span_s", C.c_double),
+                ("throughput_rps", C.c_double), ("tflops", C.c_double), ("peak_bytes", C.c_int64),
+                ("budget_bytes", C.c_int64)]
+
+
 class QSTargetProfile(C.Structure):
     _fields_ = [
         ("name", C.c_char * 64),
@@ -100,6 +108,16 @@ SIGNATURES = {
                                    C.POINTER(C.c_void_p)]),
     "qs_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "qs_graph_destroy": (C.c_int, [C.c_void_p]),
+    "qs_server_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "qs_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64,
+                                   C.c_int64, C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "qs_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "qs_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double)]),
+    "qs_server_drain": (C.c_int, [C.c_void_p]),
+    "qs_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(QSServerStats)]),
+    "qs_server_reset_stats": (C.c_int, [C.c_void_p]),
+    "qs_server_destroy": (C.c_int, [C.c_void_p]),
     "qs_fill_offsets": (C.c_int, [C.POINTER(QSGroup), C.c_int64, C.c_int64, C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64)]),
 }

@@ -67,6 +67,7 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
     g.beta_re = c[2];
     g.beta_im = c[3];
     g.beta_case = (c[2] == 0 && c[3] == 0) ? 0 : (c[2] == 1 && c[3] == 0) ? 1 : 2;
+    if (c[0] == 0 && c[1] == 0) g.K = 0;   // alpha == 0: C = beta C, A and B never read
     switch (items[lo].cls) {
       case TILE_128x64: TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::run(g, tm, tn, 0, smem); break;
       case TILE_32x64: TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::run(g, tm, tn, 0, smem); break;
@@ -257,6 +258,69 @@ static qs_status launch_all(qs_group_plan *g, hipStream_t s) {
     qs_status st = run_device(p, p->desc.a, p->desc.a_base, p->desc.b, p->desc.b_base, p->desc.c, p->desc.c_base,
                               c, c + 2, s);
     if (st != QS_OK) return st;
+  }
+  return QS_OK;
+}
+
+// One batch of members for the serving loop: descriptors already built,
+// staged through pinned host buffers owned by the caller's slot.
+qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vector<int> &cls,
+                              const std::vector<double> &coef4, BatchBuffers &B, hipStream_t s) {
+  const size_t n = args.size();
+  if (n == 0) return QS_OK;
+  if (n > (size_t)B.cap) {
+    if (B.d_items) (void)hipFree(B.d_items);
+    if (B.h_items) (void)hipHostFree(B.h_items);
+    if (B.d_coef) (void)hipFree(B.d_coef);
+    if (B.h_coef) (void)hipHostFree(B.h_coef);
+    B.d_items = nullptr; B.h_items = nullptr; B.d_coef = nullptr; B.h_coef = nullptr; B.cap = 0;
+    const size_t cap = std::max<size_t>(n, 64);
+    if (hipMalloc((void **)&B.d_items, sizeof(GroupItem) * cap) != hipSuccess ||
+        hipHostMalloc((void **)&B.h_items, sizeof(GroupItem) * cap, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void **)&B.d_coef, sizeof(double) * 4 * cap) != hipSuccess ||
+        hipHostMalloc((void **)&B.h_coef, sizeof(double) * 4 * cap, hipHostMallocDefault) != hipSuccess)
+      return fail(QS_ERR_MEMORY, "serving: batch buffers");
+    B.cap = (int)cap;
+  }
+  GroupItem *items = (GroupItem *)B.h_items;
+  std::map<std::tuple<int, int, int>, std::vector<GroupItem>> classes;
+  for (size_t i = 0; i < n; ++i) {
+    GroupItem it;
+    std::memset(&it, 0, sizeof it);
+    it.g = args[i];
+    it.cls = cls[i];
+    const TileShape ts = tile_shape(it.cls, it.g.compute_type, it.g.a_kfast, it.g.b_kfast);
+    it.tiles_m = (it.g.M + ts.bm - 1) / ts.bm;
+    it.tile_start = it.tiles_m * ((it.g.N + ts.bn - 1) / ts.bn);
+    it.index = (int32_t)i;
+    classes[std::make_tuple(it.g.compute_type, it.g.a_kfast, it.g.b_kfast)].push_back(it);
+  }
+  std::memcpy(B.h_coef, coef4.data(), sizeof(double) * 4 * n);
+  struct L { size_t off; int n; int64_t total; int ct, akf, bkf; };
+  std::vector<L> ls;
+  size_t off = 0;
+  for (auto &kv : classes) {
+    std::vector<GroupItem> &v = kv.second;
+    std::stable_sort(v.begin(), v.end(), [](const GroupItem &x, const GroupItem &y) { return x.g.K > y.g.K; });
+    int64_t start = 0;
+    for (auto &it : v) {
+      const int64_t c = it.tile_start;
+      it.tile_start = start;
+      start += c;
+    }
+    std::memcpy(items + off, v.data(), sizeof(GroupItem) * v.size());
+    ls.push_back(L{off, (int)v.size(), start, std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)});
+    off += v.size();
+  }
+  if (hipMemcpyAsync(B.d_items, B.h_items, sizeof(GroupItem) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(B.d_coef, B.h_coef, sizeof(double) * 4 * n, hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail(QS_ERR_DEVICE, "serving: batch upload");
+  for (const L &l : ls) {
+    if (l.total == 0) continue;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(l.total, 2 * (int64_t)cu_count()));
+    hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, (const GroupItem *)B.d_items + l.off, l.n, l.total, B.d_coef,
+                                  grid, s);
+    if (e != hipSuccess) return fail(QS_ERR_DEVICE, std::string("serving: launch: ") + hipGetErrorString(e));
   }
   return QS_OK;
 }

@@ -2,6 +2,8 @@
 // src/planning/plan.jl:16-45, and its ContractWorkspace, workspace.jl:24-52).
 #pragma once
 
+#include <vector>
+
 #include "qs_internal.h"
 
 struct qs_plan {
@@ -31,6 +33,16 @@ struct qs_plan {
 };
 
 namespace qs {
+// Device and pinned host buffers of one in-flight grouped batch.
+struct BatchBuffers {
+  void *d_items = nullptr, *h_items = nullptr;
+  double *d_coef = nullptr, *h_coef = nullptr;
+  int cap = 0;
+};
+// Launch `args` (non-mixed GEMM descriptors, K unsplit) as grouped launches
+// with tile classes `cls` and coefficients {are, aim, bre, bim} per member.
+qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vector<int> &cls,
+                              const std::vector<double> &coef4, BatchBuffers &B, hipStream_t s);
 int64_t group_length(const qs_group &g);
 qs_status validate_group(const qs_group &g, const char *name);
 void fill_offsets(const qs_group &g, int64_t first, int64_t count, int64_t *o0, int64_t *o1);
