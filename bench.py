@@ -84,6 +84,49 @@ def config4(qs, torch, dev):
     return plan, 2 * 4096 ** 3, 3 * 8 * 4096 ** 2, (A, B, C), 0.0
 
 
+def bench_split(qs, torch, dev, d=64, reps=5):
+    """Rank-4 all-64 F64 with entries N(0,1) * 2^u (u integer in [-20, 20] per
+    row of A and per column of B): plain FP64 MFMA vs split precision (INT8
+    slices), both timed per execute; the split error from the sampled Dot2
+    report."""
+    rng = np.random.default_rng(4)
+    SV = qs.StridedView
+    ua = rng.integers(-20, 21, size=(d, d))
+    ub = rng.integers(-20, 21, size=(d, d))
+    g = torch.Generator(device=dev)
+    g.manual_seed(4)
+    A = torch.randn((d, d, d, d), dtype=torch.float64, device=dev, generator=g)
+    B = torch.randn((d, d, d, d), dtype=torch.float64, device=dev, generator=g)
+    # column-major A[i,m,j,n]: storage index i + d m + d^2 j + d^3 n -> torch dims (n, j, m, i)
+    A *= torch.from_numpy(np.exp2(ua)).to(dev).t()[None, :, None, :]
+    B *= torch.from_numpy(np.exp2(ub)).to(dev).t()[:, None, :, None]
+    A, B = A.reshape(-1), B.reshape(-1)
+    C1 = torch.zeros(d ** 4, dtype=torch.float64, device=dev)
+    C2 = torch.zeros(d ** 4, dtype=torch.float64, device=dev)
+    st = colmajor((d,) * 4)
+    mk = lambda Cst, prec: qs.plan_contract(SV(Cst, (d,) * 4, st), SV(A, (d,) * 4, st), (1, 5, 2, 6),
+                                            SV(B, (d,) * 4, st), (5, 3, 6, 4), (1, 2, 3, 4), precision=prec)
+    plain = mk(C1, "plain")
+    split = mk(C2, "split")
+    flops = 2 * d ** 6
+    out = {}
+    for name, p in (("plain", plain), ("split", split)):
+        _, km, _ = time_plan(qs, torch, p, 0.0, reps, 2)
+        out[name + "_ms"] = round(km * 1e3, 4)
+        out[name + "_tflops_fp64_equiv"] = round(flops / km / 1e12, 2)
+    qs.set_error_report(split, True)
+    qs.execute(split, 1.0, 0.0)
+    torch.cuda.synchronize()
+    rep = qs.error_report(split)
+    diff = (C2 - C1).norm().item() / C1.norm().item()
+    out.update({"slices": rep["slices"], "split_vs_plain_speedup": round(out["plain_ms"] / out["split_ms"], 3),
+                "split_rel_fro_err_sampled_dot2": rep["rel_fro"], "split_max_rel_err_sampled": rep["max_rel"],
+                "split_vs_plain_rel_fro": diff,
+                "int8_pops_equiv": round(flops * rep["slices"] * (rep["slices"] + 1) / 2 / (out["split_ms"] * 1e-3)
+                                         / 1e15, 3)})
+    return out
+
+
 def config1(qs, torch, dev):
     """C[a,n,b] := A[a,k,b] * B[k,n], a=b=48, k=n=96, F64."""
     rng = np.random.default_rng(1)
@@ -398,6 +441,7 @@ def main():
             extra[name] = {"gflops": round(fl * steps / w / 1e9, 1), "kernel_ms": round(km * 1e3, 4),
                            "mfma_frac": round(fl / km / PEAK_FP64, 4), "plan": repr(p)}
             del p, vw
+        extra["split_rank4_all64_f64"] = bench_split(qs, torch, dev)
         extra["grouped_512_mps_f64"] = bench_grouped(qs, torch, dev, False)
         extra["grouped_512_mps_c64"] = bench_grouped(qs, torch, dev, True)
         pools = served_pools(qs, torch, dev)

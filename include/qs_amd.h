@@ -179,6 +179,25 @@ qs_status qs_plan_rebind(qs_plan *plan, const void *a, int64_t a_base, int64_t a
  * (src/execution/workspace.jl:110-123).  NULL is a no-op. */
 qs_status qs_plan_destroy(qs_plan *plan);
 
+/* ---- split precision ------------------------------------------------------
+ * No counterpart in the reference.  A plan created with precision
+ * QS_SPLIT_INT8 (real compute types) runs the Ozaki scheme on the INT8 MFMA
+ * units: exact power-of-two row / column scales, S INT8 slices per operand
+ * with S chosen from `target_rel_err` (default 1e-13 for F64, 1e-6 for F32),
+ * exact INT32 slice products, FP64 recombination.  With the report switched
+ * on, every execute also evaluates sampled entries with Dot2 (compensated
+ * dot product) on the same operands and records the delivered error. */
+typedef struct qs_error_report {
+  int32_t slices;     /* INT8 slices per operand               */
+  int32_t samples;    /* entries evaluated in the last execute  */
+  double rel_fro;     /* sampled relative Frobenius error       */
+  double max_rel;     /* largest sampled relative error         */
+  double target;      /* the requested error (0 = default)      */
+} qs_error_report;
+
+qs_status qs_plan_set_error_report(qs_plan *plan, int32_t on);
+qs_status qs_plan_get_error_report(const qs_plan *plan, qs_error_report *out);
+
 /* ---- autotuner ----------------------------------------------------------
  * No counterpart in the reference (whose blocking is an analytic cache model,
  * src/planning/blocking.jl:66-77): the GPU tile and split-K factor are chosen

@@ -11,8 +11,8 @@ from .strided import StridedView
 from .labels import (AxisGroup, affine_ramp, axis_length, classify_labels, group_ranks, leading_unit_run,
                      map_ramp_step, order_contract_labels, order_free_labels, prefer_swap)
 from .plan import (ContractPlan, MFMAKernel, PathModes, autotune, autotune_clear, autotune_load, autotune_save,
-                   compute_type, contract, contract_, execute, execute_, isconj, op_conjugates, plan_contract, planned,
-                   release)
+                   compute_type, contract, contract_, error_report, execute, execute_, isconj, op_conjugates, plan_contract, planned,
+                   release, set_error_report)
 from .backend import QuasiStridedBackend, contraction_labels, tensorcontract
 from .target import target_profile
 from .group import GroupPlan, Graph
@@ -23,7 +23,7 @@ __all__ = [
     "ArgumentError", "BoundsError", "DeviceError", "DimensionMismatch", "UnsupportedError", "StridedView",
     "AxisGroup", "affine_ramp", "axis_length", "classify_labels", "group_ranks", "leading_unit_run",
     "map_ramp_step", "order_contract_labels", "order_free_labels", "prefer_swap", "ContractPlan", "MFMAKernel",
-    "PathModes", "autotune", "autotune_clear", "autotune_load", "autotune_save", "compute_type", "contract", "contract_", "execute", "execute_", "isconj", "op_conjugates",
+    "PathModes", "autotune", "autotune_clear", "autotune_load", "autotune_save", "compute_type", "contract", "contract_", "error_report", "set_error_report", "execute", "execute_", "isconj", "op_conjugates",
     "plan_contract", "planned", "release", "QuasiStridedBackend", "contraction_labels", "tensorcontract",
     "target_profile", "GroupPlan", "Graph", "Server",
 ]

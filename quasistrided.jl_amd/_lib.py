@@ -70,6 +70,11 @@ class QSServerStats(C.Structure):
                 ("budget_bytes", C.c_int64)]
 
 
+class QSErrorReport(C.Structure):
+    _fields_ = [("slices", C.c_int32), ("samples", C.c_int32), ("rel_fro", C.c_double), ("max_rel", C.c_double),
+                ("target", C.c_double)]
+
+
 class QSTargetProfile(C.Structure):
     _fields_ = [
         ("name", C.c_char * 64),
@@ -96,6 +101,8 @@ SIGNATURES = {
     "qs_plan_rebind": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
                                  C.c_void_p, C.c_int64, C.c_int64]),
     "qs_plan_destroy": (C.c_int, [C.c_void_p]),
+    "qs_plan_set_error_report": (C.c_int, [C.c_void_p, C.c_int32]),
+    "qs_plan_get_error_report": (C.c_int, [C.c_void_p, C.POINTER(QSErrorReport)]),
     "qs_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(QSTuneResult)]),
     "qs_autotune_save": (C.c_int, [C.c_char_p]),
     "qs_autotune_load": (C.c_int, [C.c_char_p]),

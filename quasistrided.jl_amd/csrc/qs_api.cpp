@@ -242,9 +242,10 @@ qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
   else if (K == 1) p->path = QS_PATH_OUTER;
   else if (M <= 32) p->path = QS_PATH_SKINNY;
   else p->path = QS_PATH_GEMM;
-  if (d.precision == QS_SPLIT_INT8 && (p->path == QS_PATH_GEMM || p->path == QS_PATH_SKINNY)) {
+  const bool split = d.precision == QS_SPLIT_INT8 && (p->path == QS_PATH_GEMM || p->path == QS_PATH_SKINNY);
+  if (split && is_complex(ct)) {
     delete p;
-    return fail(QS_ERR_UNSUPPORTED, "plan_contract: split precision is not available in this build");
+    return fail(QS_ERR_UNSUPPORTED, "plan_contract: split precision is implemented for real compute types only");
   }
 
   // Coalescing: put a unit-stride dim first in each operand's gather
@@ -305,7 +306,12 @@ qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
   p->tile = TILE_64x64;
   p->k_splits = 1;
   p->k_chunk = K;
-  if (p->path == QS_PATH_GEMM || p->path == QS_PATH_SKINNY) {
+  if (split) {
+    p->path = QS_PATH_SPLIT;
+    p->slices = split_slices_for(d.target_rel_err, ct);
+    qs_status ss = split_prepare(p);
+    if (ss != QS_OK) { qs_plan_destroy(p); return ss; }
+  } else if (p->path == QS_PATH_GEMM || p->path == QS_PATH_SKINNY) {
     int tile = (d.tile >= 0 && d.tile < TILE_COUNT) ? d.tile : -1;
     int splits = d.k_splits > 0 ? d.k_splits : 0;
     if (tile < 0 && splits == 0) {
@@ -441,6 +447,8 @@ qs_status run_device(qs_plan *p, const void *A, int64_t a_base, const void *B, i
     e = launch_dot(g, s);
   } else if (p->path == QS_PATH_OUTER) {
     e = launch_outer(g, s);
+  } else if (p->path == QS_PATH_SPLIT) {
+    e = launch_split(p, g, p->split_report_on != 0, s);
   } else {
     e = launch_gemm(g, s);
   }
@@ -543,9 +551,27 @@ qs_status qs_plan_destroy(qs_plan *p) {
   if (!p) return QS_OK;
   if (p->d_tables) (void)hipFree(p->d_tables);
   if (p->d_partials) (void)hipFree(p->d_partials);
+  if (p->d_split) (void)hipFree(p->d_split);
   for (int i = 0; i < 3; ++i)
     if (p->stage[i]) (void)hipFree(p->stage[i]);
   delete p;
+  return QS_OK;
+}
+
+qs_status qs_plan_set_error_report(qs_plan *p, int32_t on) {
+  if (!p) return fail(QS_ERR_ARGUMENT, "qs_plan_set_error_report: null plan");
+  p->split_report_on = on ? 1 : 0;
+  return QS_OK;
+}
+
+qs_status qs_plan_get_error_report(const qs_plan *p, qs_error_report *out) {
+  if (!p || !out) return fail(QS_ERR_ARGUMENT, "qs_plan_get_error_report: null argument");
+  std::memset(out, 0, sizeof *out);
+  out->slices = p->slices;
+  out->samples = p->split_report.samples;
+  out->rel_fro = p->split_report.rel_fro;
+  out->max_rel = p->split_report.max_rel;
+  out->target = p->desc.target_rel_err;
   return QS_OK;
 }
 

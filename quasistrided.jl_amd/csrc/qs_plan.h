@@ -27,6 +27,11 @@ struct qs_plan {
   int64_t kA_lo = 0, kA_hi = 0, kB_lo = 0, kB_hi = 0;
   // Absolute element spans of the last bounds check.
   int64_t a_lo = 0, a_hi = -1, b_lo = 0, b_hi = -1, c_lo = 0, c_hi = -1;
+  // Split precision: slices, scales and the sampled error report.
+  void *d_split = nullptr;
+  int64_t split_bytes = 0, split_Mp = 0, split_Np = 0, split_Kp = 0;
+  int split_report_on = 0;
+  struct { int32_t samples; double rel_fro, max_rel; } split_report = {0, 0.0, 0.0};
   // Host-memory staging buffers (A, B, C spans) and their capacities in bytes.
   void *stage[3] = {nullptr, nullptr, nullptr};
   size_t stage_cap[3] = {0, 0, 0};
@@ -56,6 +61,11 @@ qs_status configure_gemm(qs_plan *p, int tile, int splits);
 // The autotune cache's choice for the plan's shape class, if any.
 bool tune_lookup(const qs_plan *p, int &tile, int &splits);
 int grouped_tile(int64_t M, int64_t N, int ct);
+// Split precision (qs_split.hip).
+static constexpr int kSplitSamples = 256;
+int split_slices_for(double target, int ct);
+qs_status split_prepare(qs_plan *p);
+hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t s);
 GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *B, int64_t b_base, void *C,
                    int64_t c_base);
 }  // namespace qs

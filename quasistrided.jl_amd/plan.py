@@ -174,6 +174,22 @@ def autotune(plan: ContractPlan, reps: int = 5, stream=None) -> dict:
     return {f: getattr(r, f) for f, _ in _lib.QSTuneResult._fields_ if not f.startswith("_")}
 
 
+def set_error_report(plan: ContractPlan, on: bool = True):
+    """Split-precision plans: sample-check every execute against a Dot2
+    evaluation of the same entries (costs a stream synchronization)."""
+    st = _lib.lib().qs_plan_set_error_report(plan.handle, 1 if on else 0)
+    if st != 0:
+        raise_status(st, _lib.last_error())
+
+
+def error_report(plan: ContractPlan) -> dict:
+    r = _lib.QSErrorReport()
+    st = _lib.lib().qs_plan_get_error_report(plan.handle, C.byref(r))
+    if st != 0:
+        raise_status(st, _lib.last_error())
+    return {f: getattr(r, f) for f, _ in _lib.QSErrorReport._fields_}
+
+
 def autotune_save(path: str):
     st = _lib.lib().qs_autotune_save(str(path).encode())
     if st != 0:

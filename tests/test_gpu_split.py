@@ -1,0 +1,110 @@
+"""Split precision (Ozaki scheme on INT8 MFMA) against the oracle, with the
+delivered-error report.  Tolerances: relative Frobenius <= 1e-13 (FP64
+target) and <= 1e-5 (FP32)."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import SV, brute_ref, oracle_of_plan, qs, rel_fro, storage_np, values
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def colmajor(size):
+    out, s = [], 1
+    for n in size:
+        out.append(s)
+        s *= n
+    return tuple(out)
+
+
+def rank4(rng, d, dtype=np.float64, spread=20):
+    """C[i,j,k,l] := A[i,m,j,n] * B[m,k,n,l]: entries N(0,1) * 2^u, u integer
+    in [-spread, spread] per (i,j) of A and per (k,l) of B."""
+    ua = rng.integers(-spread, spread + 1, size=(d, d))
+    ub = rng.integers(-spread, spread + 1, size=(d, d))
+    A = rng.standard_normal((d, d, d, d)) * np.exp2(ua)[:, None, :, None]
+    B = rng.standard_normal((d, d, d, d)) * np.exp2(ub)[None, :, None, :]
+    Av = SV(torch.from_numpy(A.reshape(-1, order="F").astype(dtype)).to(DEV), (d,) * 4, colmajor((d,) * 4))
+    Bv = SV(torch.from_numpy(B.reshape(-1, order="F").astype(dtype)).to(DEV), (d,) * 4, colmajor((d,) * 4))
+    Cv = SV(torch.from_numpy(rng.standard_normal(d ** 4).astype(dtype)).to(DEV), (d,) * 4, colmajor((d,) * 4))
+    return Cv, Av, Bv
+
+
+@pytest.mark.parametrize("d,beta", [(16, 0.0), (16, 0.5), (24, 0.0)])
+def test_split_f64_rank4_meets_1e13(oracle, d, beta):
+    case(f"split F64 rank-4 all {d}, 2^u row/col scaling, beta={beta}")
+    rng = np.random.default_rng(40 + d)
+    Cv, Av, Bv = rank4(rng, d)
+    c0, a0, b0 = storage_np(Cv.parent), storage_np(Av.parent), storage_np(Bv.parent)
+    plan = qs.plan_contract(Cv, Av, (1, 5, 2, 6), Bv, (5, 3, 6, 4), (1, 2, 3, 4), precision="split")
+    assert plan.path == "split" and plan.info["slices"] >= 6, plan.info
+    qs.set_error_report(plan, True)
+    qs.execute(plan, 1.25, beta)
+    torch.cuda.synchronize()
+    got = storage_np(Cv.parent)
+    want = oracle_of_plan(oracle, plan, 1.25, beta, a0, b0, c0)
+    err = rel_fro(got, want)
+    rep = qs.error_report(plan)
+    print(f"[split] d={d} slices={plan.info['slices']} err={err:.3e} report={rep}", flush=True)
+    assert err <= 1e-13, err
+    assert rep["samples"] > 0 and rep["rel_fro"] <= 1e-13, rep
+
+
+def test_split_f64_ragged_and_strided(oracle):
+    case("split F64 ragged extents (M, N, K not multiples of 64), strided operands")
+    rng = np.random.default_rng(7)
+    M, K, N = 70, 131, 95
+    A = rng.standard_normal(2 * M * K)
+    Av = SV(torch.from_numpy(A).to(DEV), (M, K), (2, 2 * M))            # every other row
+    Bv = SV(torch.from_numpy(rng.standard_normal(K * N)).to(DEV), (K, N), (N, 1))   # K-fast
+    Cv = SV(torch.from_numpy(rng.standard_normal(M * N + 5)).to(DEV), (M, N), (1, M), 5)
+    c0 = storage_np(Cv.parent)
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split")
+    assert plan.path == "split"
+    qs.execute(plan, 1.0, -0.5)
+    torch.cuda.synchronize()
+    want = oracle_of_plan(oracle, plan, 1.0, -0.5, storage_np(plan.Astorage), storage_np(plan.Bstorage), c0)
+    got = storage_np(Cv.parent)
+    assert rel_fro(got, want) <= 1e-13
+    assert np.array_equal(got[:5], c0[:5])
+
+
+def test_split_f32(oracle):
+    case("split F32 (target 1e-6)")
+    rng = np.random.default_rng(8)
+    M, K, N = 200, 300, 150
+    Av = SV(torch.from_numpy(rng.standard_normal(M * K).astype(np.float32)).to(DEV), (M, K), (1, M))
+    Bv = SV(torch.from_numpy(rng.standard_normal(K * N).astype(np.float32)).to(DEV), (K, N), (1, K))
+    Cv = SV(torch.from_numpy(np.zeros(M * N, np.float32)).to(DEV), (M, N), (1, M))
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split")
+    qs.execute(plan, 1.0, 0.0)
+    torch.cuda.synchronize()
+    ref = brute_ref(Av, (1, 2), Bv, (2, 3), np.zeros((M, N)), (1, 3), 1.0, 0.0, dtype=np.float64)
+    assert rel_fro(values(Cv), ref) <= 1e-5
+
+
+def test_split_alpha_zero_never_reads_ab():
+    case("split: alpha == 0 scales C only")
+    M, K, N = 64, 64, 64
+    Av = SV(torch.full((M * K,), float("nan"), dtype=torch.float64, device=DEV), (M, K), (1, M))
+    Bv = SV(torch.full((K * N,), float("nan"), dtype=torch.float64, device=DEV), (K, N), (1, K))
+    c0 = np.arange(M * N, dtype=np.float64)
+    Cv = SV(torch.from_numpy(c0.copy()).to(DEV), (M, N), (1, M))
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split")
+    qs.execute(plan, 0.0, 0.5)
+    torch.cuda.synchronize()
+    assert np.array_equal(storage_np(Cv.parent), 0.5 * c0)
+
+
+def test_split_complex_is_rejected():
+    Av = SV(torch.zeros(16, dtype=torch.complex128, device=DEV), (4, 4), (1, 4))
+    Bv = SV(torch.zeros(16, dtype=torch.complex128, device=DEV), (4, 4), (1, 4))
+    Cv = SV(torch.zeros(16, dtype=torch.complex128, device=DEV), (4, 4), (1, 4))
+    with pytest.raises(qs.UnsupportedError):
+        qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split")
