@@ -87,7 +87,7 @@ def config4(qs, torch, dev):
 def bench_split(qs, torch, dev, d=64, reps=5):
     """Rank-4 all-64 F64 with entries N(0,1) * 2^u (u integer in [-20, 20] per
     row of A and per column of B): plain FP64 MFMA vs split precision (INT8
-    slices), both timed per execute; the split error from the sampled Dot2
+    residue GEMMs), both timed per execute; the split error from the sampled Dot2
     report."""
     rng = np.random.default_rng(4)
     SV = qs.StridedView
@@ -119,10 +119,10 @@ def bench_split(qs, torch, dev, d=64, reps=5):
     torch.cuda.synchronize()
     rep = qs.error_report(split)
     diff = (C2 - C1).norm().item() / C1.norm().item()
-    out.update({"slices": rep["slices"], "split_vs_plain_speedup": round(out["plain_ms"] / out["split_ms"], 3),
+    out.update({"moduli": rep["moduli"], "split_vs_plain_speedup": round(out["plain_ms"] / out["split_ms"], 3),
                 "split_rel_fro_err_sampled_dot2": rep["rel_fro"], "split_max_rel_err_sampled": rep["max_rel"],
                 "split_vs_plain_rel_fro": diff,
-                "int8_pops_equiv": round(flops * rep["slices"] * (rep["slices"] + 1) / 2 / (out["split_ms"] * 1e-3)
+                "int8_pops_equiv": round(flops * rep["moduli"] / (out["split_ms"] * 1e-3)
                                          / 1e15, 3)})
     return out
 

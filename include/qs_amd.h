@@ -73,8 +73,8 @@ typedef enum qs_memory {
 /* Precision modes. */
 typedef enum qs_precision {
   QS_PLAIN = 0,       /* native FP64/FP32 MFMA in the compute type            */
-  QS_SPLIT_INT8 = 1   /* FP64/FP32 from INT8 slices of scaled operands
-                         (Ozaki scheme) on the INT8 MFMA units               */
+  QS_SPLIT_INT8 = 1   /* FP64/FP32 from INT8 residues of scaled integer
+                         operands (Ozaki scheme II, CRT) on the INT8 MFMA units */
 } qs_precision;
 
 /* Everything `ContractPlan` (src/planning/plan.jl:16-45) carries that the
@@ -108,7 +108,7 @@ typedef enum qs_path {
   QS_PATH_DOT = 3,     /* M == 1 or N == 1: bandwidth-bound reduction     */
   QS_PATH_OUTER = 4,   /* K == 1: bandwidth-bound streaming outer product */
   QS_PATH_SKINNY = 5,  /* small M: B streamed once, read in place         */
-  QS_PATH_SPLIT = 6    /* split-precision (INT8 slices) GEMM              */
+  QS_PATH_SPLIT = 6    /* split-precision (INT8 residue) GEMMs            */
 } qs_path;
 
 /* What a plan decided (for tests, logs and benchmarks). */
@@ -117,7 +117,7 @@ typedef struct qs_plan_info {
   int32_t tile_m, tile_n, tile_k;
   int32_t k_splits;      /* K partitions reduced in a fixed order (1 = none) */
   int32_t a_kfast, b_kfast;  /* operand gathers coalesced along K (1) or along M/N (0) */
-  int32_t slices;        /* split precision: INT8 slices per operand      */
+  int32_t moduli;        /* split precision: CRT moduli (INT8 GEMMs)      */
   int64_t m, n, k;       /* axis lengths */
   int64_t workspace_bytes;
 } qs_plan_info;
@@ -181,14 +181,17 @@ qs_status qs_plan_destroy(qs_plan *plan);
 
 /* ---- split precision ------------------------------------------------------
  * No counterpart in the reference.  A plan created with precision
- * QS_SPLIT_INT8 (real compute types) runs the Ozaki scheme on the INT8 MFMA
- * units: exact power-of-two row / column scales, S INT8 slices per operand
- * with S chosen from `target_rel_err` (default 1e-13 for F64, 1e-6 for F32),
- * exact INT32 slice products, FP64 recombination.  With the report switched
- * on, every execute also evaluates sampled entries with Dot2 (compensated
- * dot product) on the same operands and records the delivered error. */
+ * QS_SPLIT_INT8 (real compute types) runs Ozaki scheme II on the INT8 MFMA
+ * units: A and B scaled row / column-wise to integers whose norms bound every
+ * entry of the integer product below P / 2, one exact INT8 GEMM per modulus
+ * of P (pairwise coprime moduli <= 256, their number chosen from
+ * `target_rel_err` -- default 1e-13 for F64, 1e-6 for F32 -- and K), and
+ * FP64 reconstruction by the Chinese remainder theorem.  With the report
+ * switched on, every execute also evaluates sampled entries with Dot2
+ * (compensated dot product) on the same operands and records the delivered
+ * error. */
 typedef struct qs_error_report {
-  int32_t slices;     /* INT8 slices per operand               */
+  int32_t moduli;     /* CRT moduli (INT8 GEMMs)               */
   int32_t samples;    /* entries evaluated in the last execute  */
   double rel_fro;     /* sampled relative Frobenius error       */
   double max_rel;     /* largest sampled relative error         */

@@ -52,7 +52,7 @@ class QSContractDesc(C.Structure):
 class QSPlanInfo(C.Structure):
     _fields_ = [
         ("path", C.c_int32), ("tile_m", C.c_int32), ("tile_n", C.c_int32), ("tile_k", C.c_int32),
-        ("k_splits", C.c_int32), ("a_kfast", C.c_int32), ("b_kfast", C.c_int32), ("slices", C.c_int32),
+        ("k_splits", C.c_int32), ("a_kfast", C.c_int32), ("b_kfast", C.c_int32), ("moduli", C.c_int32),
         ("m", C.c_int64), ("n", C.c_int64), ("k", C.c_int64), ("workspace_bytes", C.c_int64),
     ]
 
@@ -71,7 +71,7 @@ class QSServerStats(C.Structure):
 
 
 class QSErrorReport(C.Structure):
-    _fields_ = [("slices", C.c_int32), ("samples", C.c_int32), ("rel_fro", C.c_double), ("max_rel", C.c_double),
+    _fields_ = [("moduli", C.c_int32), ("samples", C.c_int32), ("rel_fro", C.c_double), ("max_rel", C.c_double),
                 ("target", C.c_double)]
 
 

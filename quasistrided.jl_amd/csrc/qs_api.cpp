@@ -284,6 +284,7 @@ qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
   fill_offsets(mg, 0, M, mA, mC);
   fill_offsets(ng, 0, N, nB, nC);
   fill_offsets(kg, 0, K, kA, kB);
+  p->c_mfast = (M > 1 && mC[1] - mC[0] == 1) || !(N > 1 && nC[1] - nC[0] == 1);
   hipError_t e = hipSuccess;
   if (nt > 0) {
     e = hipMalloc(&p->d_tables, sizeof(int64_t) * (size_t)nt);
@@ -308,7 +309,7 @@ qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
   p->k_chunk = K;
   if (split) {
     p->path = QS_PATH_SPLIT;
-    p->slices = split_slices_for(d.target_rel_err, ct);
+    p->moduli = split_moduli_for(d.target_rel_err, ct, K);
     qs_status ss = split_prepare(p);
     if (ss != QS_OK) { qs_plan_destroy(p); return ss; }
   } else if (p->path == QS_PATH_GEMM || p->path == QS_PATH_SKINNY) {
@@ -512,7 +513,7 @@ qs_status qs_plan_get_info(const qs_plan *p, qs_plan_info *out) {
   out->k_splits = p->k_splits;
   out->a_kfast = p->a_kfast;
   out->b_kfast = p->b_kfast;
-  out->slices = p->slices;
+  out->moduli = p->moduli;
   out->m = p->M;
   out->n = p->N;
   out->k = p->K;
@@ -567,7 +568,7 @@ qs_status qs_plan_set_error_report(qs_plan *p, int32_t on) {
 qs_status qs_plan_get_error_report(const qs_plan *p, qs_error_report *out) {
   if (!p || !out) return fail(QS_ERR_ARGUMENT, "qs_plan_get_error_report: null argument");
   std::memset(out, 0, sizeof *out);
-  out->slices = p->slices;
+  out->moduli = p->moduli;
   out->samples = p->split_report.samples;
   out->rel_fro = p->split_report.rel_fro;
   out->max_rel = p->split_report.max_rel;

@@ -15,8 +15,9 @@ struct qs_plan {
   int k_splits = 1;
   int64_t k_chunk = 0;
   int a_kfast = 0, b_kfast = 0;
+  int c_mfast = 1;             // C's unit-stride side: M (1) or N (0)
   int mixed = 0;
-  int slices = 0;
+  int moduli = 0;             // split precision: CRT moduli (INT8 GEMMs)
   int64_t workspace_bytes = 0;
   // Device offset tables: mA mC nB nC kA kB, one allocation.
   int64_t *d_tables = nullptr;
@@ -27,7 +28,7 @@ struct qs_plan {
   int64_t kA_lo = 0, kA_hi = 0, kB_lo = 0, kB_hi = 0;
   // Absolute element spans of the last bounds check.
   int64_t a_lo = 0, a_hi = -1, b_lo = 0, b_hi = -1, c_lo = 0, c_hi = -1;
-  // Split precision: slices, scales and the sampled error report.
+  // Split precision: residues, scales and the sampled error report.
   void *d_split = nullptr;
   int64_t split_bytes = 0, split_Mp = 0, split_Np = 0, split_Kp = 0;
   int split_report_on = 0;
@@ -63,7 +64,8 @@ bool tune_lookup(const qs_plan *p, int &tile, int &splits);
 int grouped_tile(int64_t M, int64_t N, int ct);
 // Split precision (qs_split.hip).
 static constexpr int kSplitSamples = 256;
-int split_slices_for(double target, int ct);
+int split_moduli_for(double target, int ct, int64_t K);
+const std::vector<int> &split_moduli_table();
 qs_status split_prepare(qs_plan *p);
 hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t s);
 GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *B, int64_t b_base, void *C,

@@ -1,283 +1,504 @@
-// Split-precision FP64 / FP32 contraction on the INT8 MFMA units (Ozaki
-// scheme, Ozaki-Ogita-Oishi-Rump 2012, with integer slices as in the
-// int8-emulation literature).
+// Split-precision FP64 / FP32 contraction on the INT8 MFMA units by the
+// Chinese remainder theorem (Ozaki scheme II: Ozaki, Uchino, Imamura, "Ozaki
+// Scheme II: A GEMM-oriented emulation of floating-point matrix
+// multiplication using an integer modular technique", 2025).
 //
-//   1. Exact power-of-two scales: e_i = exponent of max_k |A[i,k]| (rows of
-//      A), f_j = exponent of max_k |B[k,j]| (columns of B), so every scaled
-//      entry lies in (-1, 1).  Scaling by 2^-e is exact.
-//   2. S slices per operand: t = x * 2^7, sigma = trunc(t), x <- t - sigma,
-//      repeated S times; x = sum_p sigma_p 2^(-7p) + O(2^(-7S)), |sigma| <= 127.
-//      Every step is exact in FP64.  Slices are stored dense, K contiguous,
-//      rows and K zero-padded to multiples of 64.
-//   3. C_scaled = sum_{p+q <= S+1} 2^(-7(p+q)) (A_p B_q): each slice product
-//      on v_mfma_i32_16x16x64_i8 with exact INT32 accumulation (the pairs of
-//      one diagonal d = p+q share an INT32 accumulator; flushed to FP64 every
-//      8192 K steps so it never overflows), folded into FP64 registers.
-//   4. Epilogue: C = alpha * 2^(e_i + f_j) C_scaled + beta * C through C's
-//      offset tables, one rounding to C's type.
+//   1. Scales.  For every row i of A (over K) and column j of B the 2-norm
+//      n_i (scaled sum of squares, full double range) gives a scale
+//      s_i = T / n_i, kept as a mantissa q_i in (0.5, 2] and a power of two.
+//      A'_ik = rint(A_ik s_i) and B'_kj = rint(B_kj t_j) are integers whose
+//      row / column norms are at most T (+ sqrt(K) / 2 from the rounding), so
+//      by Cauchy-Schwarz every integer product C'_ij = sum_k A'_ik B'_kj has
+//      |C'_ij| < P / 2, with P = m_1 ... m_n a product of pairwise coprime
+//      moduli <= 256.  No bound depends on K.
+//   2. Residues.  For every modulus m_l: A'_ik y_l mod m_l and B'_kj mod m_l
+//      in the symmetric range [-m/2, m/2), which is INT8 (y_l = (P/m_l)^-1
+//      mod m_l, folded into A).  A' is split into two base-2^32 digits so
+//      every residue is computed exactly in FP64 with a correctly rounded
+//      symmetric quotient (no fix-up steps).  Residues are
+//      stored in MFMA fragment order: 16 rows x 64 K per 1 KiB block, lane l
+//      holding row l % 16, K bytes 16 (l / 16) .. +15.
+//   3. n INT8 GEMMs on v_mfma_i32_16x16x64_i8 with exact INT32 accumulation
+//      (reduced mod m every 4096 K, so K is unbounded); the epilogue stores
+//      U_l = acc mod m_l as a byte.
+//   4. Reconstruction.  C' = sum_l U_l Q_l - q P with Q_l = P / m_l and
+//      q = rint(sum_l U_l Q_l / P).  Q_l = Q1_l + Q2_l where Q1_l is a
+//      multiple of 2^g short enough that sum_l U_l Q1_l and q P1 are exact in
+//      FP64, so C' comes out to within a few units of its last bit.  Then
+//      C = alpha C' / (s_i t_j) + beta C through C's offset tables, one
+//      rounding to C's type.
 //
-// S is chosen from the requested relative error (qs_contract_desc
-// target_rel_err); the sampled error report (Dot2 of Ogita-Rump-Oishi 2005
-// on the same gathered operands) measures what was delivered.
+// The number of moduli follows from the requested relative error (default
+// 1e-13 for F64, 1e-6 for F32) and K; the sampled error report (Dot2 of
+// Ogita-Rump-Oishi 2005 on the same gathered operands) measures what was
+// delivered.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <string>
+#include <vector>
 
 #include "qs_gemm_impl.h"
 #include "qs_plan.h"
-
-#include <vector>
 
 namespace qs {
 
 typedef int qs_v4i __attribute__((ext_vector_type(4)));
 
-// One operand's gather for the prep kernels: R rows (M of A, N of B) by K.
-struct SplitOperand {
-  const void *X;
-  int type;
-  int kfast;
-  int64_t base;
-  const int64_t *rowtab, *ktab;
-  int64_t R, K;
-  unsigned long long *rowmax;   // bit pattern of max |x| per row (>= 0 doubles order as integers)
-  int *exps;                    // per row: e with max * 2^-e in [0.5, 1)
-  int8_t *slices;               // [S][Rp][Kp]
-  int64_t Rp, Kp;
-  int S;
+static constexpr int kMaxModuli = 15;    // (n + 1) P < 2^128
+static constexpr int kNormChunk = 256;   // K per partial sum of squares
+static constexpr int kTile = 256;        // GEMM workgroup tile (M and N)
+
+// Scaled sum of squares: value s * 2^(2e).
+struct Ssq {
+  double s;
+  int e, pad;
 };
 
-__global__ __launch_bounds__(256) void split_zero_kernel(unsigned long long *p, int64_t n) {
-  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0ull;
+struct RowScale {
+  double iq;   // 1 / q_i (NaN for a row holding Inf / NaN)
+  int sh;      // power of two that undoes the scale
+  int pad;
+};
+
+// Residue weights and reconstruction constants (host-computed, passed by value).
+struct CrtConst {
+  int n;
+  double mf[kMaxModuli], invm[kMaxModuli];
+  double cst[kMaxModuli];                 // 0.5 - 128 / m
+  double w[2][kMaxModuli][2];             // digit weights 1, 2^32 mod m: [0] A (y folded in), [1] B
+  double q1[kMaxModuli], q2[kMaxModuli];
+  double p1, p2, invp;
+};
+
+struct CrtOperand {
+  const void *X;
+  int type, kfast;
+  int64_t base;
+  const int64_t *rowtab, *ktab;
+  int64_t R, K, Rp, Kp;   // rows (M of A, N of B), K, padded to 256 and 128
+  int nkc;                // K chunks of the norm pass
+  int side;               // 0: A, 1: B
+  Ssq *part;              // [nkc][R]
+  RowScale *scale;        // [R]
+  int8_t *res;            // [n][Rp/16][Kp/64][1024]
+  double Tm;              // T = Tm * 2^eT
+  int eT;
+};
+
+// Branch-free: Inf / NaN make s Inf / NaN, which marks the row.
+__device__ __forceinline__ void ssq_add(Ssq &a, double x) {
+  const double ax = fabs(x);
+  const int ex = (ax > 0.0 && ax <= DBL_MAX) ? ilogb(ax) : -2000;
+  const int ne = max(a.e, ex);
+  a.s = ldexp(a.s, 2 * (a.e - ne));
+  a.e = ne;
+  const double y = ldexp(ax, -ne);
+  a.s = __builtin_fma(y, y, a.s);
 }
 
-__device__ __forceinline__ double split_load(const SplitOperand &o, int64_t r, int64_t k) {
-  double re, im;
-  load_elem<double, false, true>(o.X, o.type, o.base + o.rowtab[r] + o.ktab[k], re, im);
-  return re;
+__device__ __forceinline__ Ssq ssq_merge(Ssq a, Ssq b) {
+  Ssq r;
+  r.e = max(a.e, b.e);
+  r.s = ldexp(a.s, 2 * (a.e - r.e)) + ldexp(b.s, 2 * (b.e - r.e));
+  r.pad = 0;
+  return r;
 }
 
-// Tile of 64 rows x 64 K per workgroup; lanes along the operand's
-// unit-stride side.  Element (tr, tk) of the tile for pass `it` (0..15).
-__device__ __forceinline__ void split_coord(int kfast, int t, int it, int &tr, int &tk) {
-  const int e = t + 256 * it;
-  if (kfast) { tk = e % 64; tr = e / 64; } else { tr = e % 64; tk = e / 64; }
-}
-
-__global__ __launch_bounds__(256) void split_rowmax_kernel(SplitOperand o) {
-  __shared__ unsigned long long smax[64];
-  const int t = threadIdx.x;
-  if (t < 64) smax[t] = 0ull;
-  __syncthreads();
-  const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
-  for (int it = 0; it < 16; ++it) {
-    int tr, tk;
-    split_coord(o.kfast, t, it, tr, tk);
-    const int64_t r = r0 + tr, k = k0 + tk;
-    if (r < o.R && k < o.K) {
-      const double v = fabs(split_load(o, r, k));
-      atomicMax(&smax[tr], (unsigned long long)__double_as_longlong(v));
+// Pass 1: partial sums of squares of 64 rows over one K chunk of 256, in a
+// fixed order (the scales, hence the results, are reproducible bit for bit).
+// Loads are unconditional and issued 16 at a time: a lane outside the
+// operand reads its first element and discards it.
+__global__ __launch_bounds__(256) void crt_norm_kernel(CrtOperand o) {
+  __shared__ Ssq red[4][64];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  const int64_t kc0 = (int64_t)blockIdx.y * kNormChunk;
+  const int64_t safe = o.base + o.rowtab[0] + o.ktab[0];
+  if (o.kfast) {
+    // Lanes along K: row wave + 4 it for every lane of the wave.
+    Ssq acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = Ssq{0.0, -1100, 0};
+#pragma unroll 1
+    for (int j = 0; j < kNormChunk / 64; ++j) {
+      const int64_t k = kc0 + 64 * j + lane;
+      const bool kin = k < o.K;
+      const int64_t ko = o.ktab[kin ? k : 0];
+      double x[16];
+#pragma unroll
+      for (int it = 0; it < 16; ++it) {
+        const int64_t r = r0 + wave + 4 * it;
+        const bool ok = kin && r < o.R;
+        double im;
+        load_elem<double, false, true>(o.X, o.type, ok ? o.base + o.rowtab[r] + ko : safe, x[it], im);
+        x[it] = ok ? x[it] : 0.0;
+      }
+#pragma unroll
+      for (int it = 0; it < 16; ++it) ssq_add(acc[it], x[it]);
+    }
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+      Ssq a = acc[it];
+      for (int off = 32; off >= 1; off >>= 1) {   // fixed butterfly; lane 0's result is used
+        Ssq b;
+        b.s = __shfl_xor(a.s, off, 64);
+        b.e = __shfl_xor(a.e, off, 64);
+        a = ssq_merge(a, b);
+      }
+      const int64_t r = r0 + wave + 4 * it;
+      if (lane == 0 && r < o.R) o.part[(int64_t)blockIdx.y * o.R + r] = a;
+    }
+  } else {
+    // Lanes along the rows: each thread owns row `lane`, K steps wave + 4 i.
+    Ssq acc = Ssq{0.0, -1100, 0};
+    const int64_t r = r0 + lane;
+    const bool rin = r < o.R;
+    const int64_t rbase = o.base + o.rowtab[rin ? r : 0];
+#pragma unroll 1
+    for (int j = 0; j < kNormChunk / 64; ++j) {
+      double x[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int64_t k = kc0 + wave + 4 * (16 * j + i);
+        const bool ok = rin && k < o.K;
+        double im;
+        load_elem<double, false, true>(o.X, o.type, ok ? rbase + o.ktab[k] : safe, x[i], im);
+        x[i] = ok ? x[i] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) ssq_add(acc, x[i]);
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (t < 64) {
+      Ssq a = red[0][t];
+      for (int w = 1; w < 4; ++w) a = ssq_merge(a, red[w][t]);
+      if (r0 + t < o.R) o.part[(int64_t)blockIdx.y * o.R + r0 + t] = a;
     }
   }
-  __syncthreads();
-  if (t < 64 && r0 + t < o.R && smax[t]) atomicMax(&o.rowmax[r0 + t], smax[t]);
 }
 
-template <int S>
-__global__ __launch_bounds__(256) void split_slice_kernel(SplitOperand o) {
-  __shared__ int8_t sl[S][64][64 + 4];
-  __shared__ int sexp[64];
+// Pass 2: scale, round and reduce one 64-row x 64-K tile to one residue
+// fragment column per modulus.
+__global__ __launch_bounds__(256) void crt_residue_kernel(CrtOperand o, const CrtConst c) {
+  __shared__ double tile[64][65];
+  __shared__ double sq[64];
+  __shared__ int ssh[64];
   const int t = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
   if (t < 64) {
     const int64_t r = r0 + t;
-    int e = 0;
+    double qv = 0.0, iq = 1.0;
+    int sh = 0;
     if (r < o.R) {
-      const double m = __longlong_as_double((long long)o.rowmax[r]);
-      if (m > 0) (void)frexp(m, &e);
-      if (blockIdx.y == 0) o.exps[r] = e;
+      Ssq a = o.part[r];
+      for (int cc = 1; cc < o.nkc; ++cc) a = ssq_merge(a, o.part[(int64_t)cc * o.R + r]);
+      if (!(a.s <= DBL_MAX)) {
+        iq = __builtin_nan("");
+      } else if (a.s > 0.0) {
+        int en0;
+        const double f = frexp(sqrt(a.s), &en0);   // norm = f 2^(en0 + e), f in [0.5, 1)
+        qv = o.Tm / f;
+        iq = f / o.Tm;
+        sh = o.eT - (en0 + a.e);
+      }
+      if (blockIdx.y == 0) o.scale[r] = RowScale{iq, -sh, 0};
     }
-    sexp[t] = e;
+    sq[t] = qv;
+    ssh[t] = sh;
   }
   __syncthreads();
-  for (int it = 0; it < 16; ++it) {
-    int tr, tk;
-    split_coord(o.kfast, t, it, tr, tk);
-    const int64_t r = r0 + tr, k = k0 + tk;
-    double x = (r < o.R && k < o.K) ? ldexp(split_load(o, r, k), -sexp[tr]) : 0.0;
+  // 16 independent loads per thread, then the scaling.
+  double x[16];
+  if (o.kfast) {
+    // Lanes along K (tk = lane), rows wave + 4 it.
+    const int64_t k = k0 + (t & 63);
+    const bool kin = k < o.K;
+    const int64_t ko = kin ? o.ktab[k] : 0;
 #pragma unroll
-    for (int p = 0; p < S; ++p) {
-      const double tt = x * 128.0;
-      const double sg = trunc(tt);
-      sl[p][tr][tk] = (int8_t)(int)sg;
-      x = tt - sg;
+    for (int it = 0; it < 16; ++it) {
+      const int64_t r = r0 + wave + 4 * it;
+      double im;
+      x[it] = 0.0;
+      if (kin && r < o.R) load_elem<double, false, true>(o.X, o.type, o.base + o.rowtab[r] + ko, x[it], im);
     }
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+      const int tr = wave + 4 * it;
+      // |x 2^sh| <= f 2^eT < T: no overflow; qv = 0 zeroes empty and padding rows.
+      tile[tr][t & 63] = rint(ldexp(x[it], ssh[tr]) * sq[tr]);
+    }
+  } else {
+    // Lanes along the rows (tr = lane), K steps wave + 4 it.
+    const int64_t r = r0 + (t & 63);
+    const bool rin = r < o.R;
+    const int64_t rbase = rin ? o.base + o.rowtab[r] : 0;
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+      const int64_t k = k0 + wave + 4 * it;
+      double im;
+      x[it] = 0.0;
+      if (rin && k < o.K) load_elem<double, false, true>(o.X, o.type, rbase + o.ktab[k], x[it], im);
+    }
+    const int tr = t & 63;
+    const double qv = sq[tr];
+    const int sh = ssh[tr];
+#pragma unroll
+    for (int it = 0; it < 16; ++it) tile[tr][wave + 4 * it] = rint(ldexp(x[it], sh) * qv);
   }
   __syncthreads();
-  // Write out K-contiguous: each thread stores 4 consecutive bytes of a row.
-  for (int p = 0; p < S; ++p)
-    for (int it = 0; it < 4; ++it) {
-      const int e = t + 256 * it;          // 1024 words: 64 rows x 16 words
-      const int tr = e / 16, w = e % 16;
-      const int64_t r = r0 + tr;
-      if (r >= o.Rp) continue;
-      const int64_t kk = k0 + 4 * w;
-      int word;
-      __builtin_memcpy(&word, &sl[p][tr][4 * w], 4);
-      *(QS_GLOBAL int *)(o.slices + ((int64_t)p * o.Rp + r) * o.Kp + kk) = word;
+  const int f = t >> 6, lane = t & 63;
+  const int row = 16 * f + (lane & 15), kk = 16 * (lane >> 4);
+  // Two digits: x = d1 2^32 + d0, d0 in [0, 2^32), |d1| < 2^31.
+  double d0[16], d1[16];
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const double xv = tile[row][kk + v];
+    d1[v] = floor(xv * 0x1p-32);
+    d0[v] = __builtin_fma(d1[v], -0x1p32, xv);
+  }
+  const int64_t RB = o.Rp / 16, KB = o.Kp / 64;
+  const int64_t frag = (r0 / 16 + f) * KB + k0 / 64;
+  for (int l = 0; l < c.n; ++l) {
+    const double m = c.mf[l], im = c.invm[l], cst = c.cst[l];
+    const double w0 = c.w[o.side][l][0], w1 = c.w[o.side][l][1];
+    int word[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      unsigned int packed = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int v = 4 * q + b;
+        // s = x w + 128 exactly (|s| < 2^41).  The symmetric quotient
+        // floor((s - 128) / m + 1/2) is exact: for odd m that value is never
+        // within 1 / (2m) of an integer, and for m = 256 every step is exact.
+        const double sv = __builtin_fma(d1[v], w1, __builtin_fma(d0[v], w0, 128.0));
+        const double qq = floor(__builtin_fma(sv, im, cst));
+        const double u = __builtin_fma(-qq, m, sv);    // residue + 128, in [0, 256)
+        packed = __builtin_amdgcn_cvt_pk_u8_f32((float)u, b, packed);
+      }
+      word[q] = (int)(packed ^ 0x80808080u);            // two's complement of the residue
+    }
+    const qs_v4i out = {word[0], word[1], word[2], word[3]};
+    *(QS_GLOBAL qs_v4i *)(o.res + ((int64_t)l * RB * KB + frag) * 1024 + lane * 16) = out;
+  }
+}
+
+// The residue GEMMs: one 256 x 256 tile of one modulus per workgroup, 8 waves
+// (2 x 4, each 128 x 64 = 8 x 4 MFMA blocks), K in stages of one fragment
+// (64 bytes): 16 A and 16 B row blocks, 32 KiB per stage, double-buffered in
+// LDS.  Two register sets alternate as the prefetch, so each stage's global
+// loads are issued two stages before they are written to LDS.  Every global
+// load, LDS write and LDS read is a contiguous 1 KiB per wave.
+struct CrtGemmArgs {
+  const int8_t *ra, *rb;
+  unsigned int *u;           // [Mp/16][Np/16][n][64]
+  int64_t MB, NB, KB;        // Mp/16, Np/16, Kp/64
+  int64_t tiles_m, tiles_n;
+  int n;
+  int m[kMaxModuli];
+  float invm[kMaxModuli];
+};
+
+__device__ __forceinline__ int crt_mod(int x, int m, float invm) {
+  // |x| < 2^27: the float quotient is off by at most one.
+  const int q = (int)floorf((float)x * invm);
+  int r = x - q * m;
+  r = r < 0 ? r + m : r;
+  r = r >= m ? r - m : r;
+  return r;
+}
+
+// WM x WN waves, each TI x TJ blocks of 16 x 16: a (16 WM TI) x (16 WN TJ)
+// tile of one modulus per workgroup.
+template <int WM, int WN, int TI, int TJ, int MINB>
+__global__ __launch_bounds__(64 * WM * WN, MINB) void crt_gemm_kernel(const CrtGemmArgs a) {
+  constexpr int NW = WM * WN, FA = WM * TI, FB = WN * TJ, SLOTS = FA + FB, NLD = SLOTS / NW;
+  static_assert(SLOTS % NW == 0, "stage slots must split evenly over the waves");
+  __shared__ __attribute__((aligned(16))) int8_t smem[2][SLOTS * 1024];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wm = wave % WM, wn = wave / WM;
+  const int64_t tiles = a.tiles_m * a.tiles_n;
+  const int64_t bid = xcd_swizzle(blockIdx.x, (int64_t)gridDim.x);
+  const int l = (int)(bid / tiles);
+  const int64_t tt = bid % tiles;
+  // Groups of four tile rows walk the N tiles together (A panels stay in L2).
+  const int64_t grp = tt / (4 * a.tiles_n), in = tt % (4 * a.tiles_n);
+  const int64_t rows = min((int64_t)4, a.tiles_m - 4 * grp);
+  const int64_t tm = 4 * grp + in % rows, tn = in / rows;
+  const int64_t mb0 = tm * FA, nb0 = tn * FB;
+  const int8_t *A = a.ra + ((int64_t)l * a.MB + mb0) * a.KB * 1024 + lane * 16;
+  const int8_t *B = a.rb + ((int64_t)l * a.NB + nb0) * a.KB * 1024 + lane * 16;
+  const int m = a.m[l];
+  const float invm = a.invm[l];
+  const int64_t KB = a.KB;
+
+  // Stage slot q = wave + NW i (i < NLD): slots below FA are A row blocks,
+  // the rest B; slot q sits at q KiB of the LDS buffer.
+  auto load = [&](qs_v4i (&r)[NLD], int64_t kb) {
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      const int q = wave + NW * i;
+      const int8_t *p = q < FA ? A + ((int64_t)q * KB + kb) * 1024 : B + ((int64_t)(q - FA) * KB + kb) * 1024;
+      r[i] = *(const QS_GLOBAL qs_v4i *)p;
+    }
+  };
+  auto put = [&](const qs_v4i (&r)[NLD], int buf) {
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) *(qs_v4i *)(smem[buf] + (wave + NW * i) * 1024 + lane * 16) = r[i];
+  };
+
+  qs_v4i acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = qs_v4i{0, 0, 0, 0};
+
+  // Iteration kb: refill `cur` (which held stage kb) with stage kb + 2,
+  // multiply stage kb out of LDS buffer kb & 1, then store `nxt` (stage
+  // kb + 1, loaded one iteration earlier) into the other buffer.  Loads and
+  // stores are unconditional (the last two stages are reloaded and stored
+  // unused) so the compiler's wait counts stay exact: each stage's loads
+  // have two iterations to land.
+  auto step = [&](int64_t kb, qs_v4i (&cur)[NLD], const qs_v4i (&nxt)[NLD]) {
+    load(cur, kb + 2 < KB ? kb + 2 : KB - 1);
+    __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the MFMAs
+    const int8_t *st = smem[kb & 1];
+    qs_v4i af[TI], bf[TJ];
+#pragma unroll
+    for (int i = 0; i < TI; ++i) af[i] = *(const qs_v4i *)(st + (wm * TI + i) * 1024 + lane * 16);
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) bf[j] = *(const qs_v4i *)(st + (FA + wn * TJ + j) * 1024 + lane * 16);
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+    if (((kb + 1) & 63) == 0 && kb + 1 < KB) {
+      // Every 4096 K: |acc| <= 4096 * 128^2 + m < 2^27 before, in [0, m) after.
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[i][j][r] = crt_mod(acc[i][j][r], m, invm);
+    }
+    put(nxt, (int)((kb + 1) & 1));
+    __syncthreads();
+  };
+
+  qs_v4i r0[NLD], r1[NLD];
+  load(r0, 0);
+  load(r1, 1);
+  put(r0, 0);
+  __syncthreads();
+  for (int64_t kb = 0; kb < KB; kb += 2) {   // KB is even
+    step(kb, r0, r1);
+    step(kb + 1, r1, r0);
+  }
+  // Lane l holds rows 4 (l / 16) + r, column l % 16 of each 16 x 16 block:
+  // one packed word per block, 256 contiguous bytes per wave.
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+      unsigned int w = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) w |= (unsigned int)crt_mod(acc[i][j][r], m, invm) << (8 * r);
+      const int64_t blk = (mb0 + wm * TI + i) * a.NB + nb0 + wn * TJ + j;
+      ((QS_GLOBAL unsigned int *)a.u)[(blk * a.n + l) * 64 + lane] = w;
     }
 }
 
-// The slice GEMM: 64 x 64 tile per workgroup (4 waves, 2 x 2, each 32 x 32 =
-// 2 x 2 MFMA blocks), K in stages of 64 bytes, all S slices of A and B of a
-// stage in LDS (row pitch 80 B keeps the 16-byte fragment reads conflict
-// free), double-buffered with a register prefetch of the next stage.
-struct SplitGemmArgs {
-  GemmArgs g;                       // C, its tables, M, N, alpha/beta, types
-  const int8_t *as, *bs;            // [S][Mp][Kp], [S][Np][Kp]
-  const int *ea, *eb;
-  int64_t Mp, Np, Kp;
+// Reconstruction and the alpha / beta store.  One wave per strip of four
+// 16 x 16 blocks laid along C's unit-stride side (64 x 16 when M is fast,
+// 16 x 64 when N is), turned through LDS so each store covers 64 consecutive
+// elements of C.
+struct CrtRecArgs {
+  GemmArgs g;
+  const unsigned int *u;
+  const RowScale *sa, *sb;
+  int64_t MB, NB;
 };
 
-template <int S>
-struct SplitTile {
-  static constexpr int BM = 64, BN = 64, BK = 64, PITCH = 80;
-  static constexpr int STAGE = S * (BM + BN) * PITCH;    // bytes
-  static constexpr int WORDS = S * (BM + BN) * BK / 16;  // 16-byte words per stage
-  static constexpr int NW = WORDS / 256;                 // per thread
-  // Two LDS stages while they fit beside the runtime's reservation; from 8
-  // slices on, one stage refilled from the register prefetch.
-  static constexpr int STAGES = 2 * STAGE <= 150 * 1024 ? 2 : 1;
-};
-
-template <int S>
-__global__ __launch_bounds__(256, 1) void split_gemm_kernel(const SplitGemmArgs a) {
-  typedef SplitTile<S> T;
-  __shared__ __attribute__((aligned(16))) int8_t smem[T::STAGES * T::STAGE];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  const int fr = lane & 15, fk = lane >> 4;
-  const int64_t tiles_m = a.Mp / 64;
-  const int64_t bid = xcd_swizzle(blockIdx.x, (int64_t)gridDim.x);
-  const int64_t m0 = (bid % tiles_m) * 64, n0 = (bid / tiles_m) * 64;
-
-  // Stage word w (16 bytes): operand (A rows 0..63, then B rows), slice, row, k-word.
-  auto src = [&](int w, int64_t k0) -> const QS_GLOBAL qs_v4i * {
-    const int kw = w & 3;               // 4 words of 16 bytes per 64-byte row
-    const int rowid = (w >> 2) % 128;   // 0..63 A, 64..127 B
-    const int p = (w >> 2) / 128;
-    const int8_t *base = rowid < 64 ? a.as + ((int64_t)p * a.Mp + m0 + rowid) * a.Kp
-                                    : a.bs + ((int64_t)p * a.Np + n0 + rowid - 64) * a.Kp;
-    return (const QS_GLOBAL qs_v4i *)(base + k0 + 16 * kw);
-  };
-  auto dst = [&](int8_t *stage, int w) -> qs_v4i * {
-    const int kw = w & 3, rowid = (w >> 2) % 128, p = (w >> 2) / 128;
-    return (qs_v4i *)(stage + ((p * 128) + rowid) * T::PITCH + 16 * kw);
-  };
-
-  qs_v4i pre[T::NW];
-  qs_v4i acc[S][2][2];
-  double accd[2][2][4];
-#pragma unroll
-  for (int d = 0; d < S; ++d)
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) acc[d][i][j] = qs_v4i{0, 0, 0, 0};
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) accd[i][j][r] = 0.0;
-
-  auto flush = [&]() {
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          double s = 0.0;
-#pragma unroll
-          for (int d = S - 1; d >= 0; --d) s += ldexp((double)acc[d][i][j][r], -7 * (d + 2));   // small first
-          accd[i][j][r] += s;
-        }
-#pragma unroll
-    for (int d = 0; d < S; ++d)
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[d][i][j] = qs_v4i{0, 0, 0, 0};
-  };
-
-  const int64_t nk = a.Kp / 64;
-#pragma unroll
-  for (int i = 0; i < T::NW; ++i) pre[i] = *src(t + 256 * i, 0);
-#pragma unroll
-  for (int i = 0; i < T::NW; ++i) *dst(smem, t + 256 * i) = pre[i];
-  __syncthreads();
-  int buf = 0;
-  for (int64_t kc = 0; kc < nk; ++kc) {
-    const bool more = kc + 1 < nk;
-    if (more) {
-#pragma unroll
-      for (int i = 0; i < T::NW; ++i) pre[i] = *src(t + 256 * i, (kc + 1) * 64);
-    }
-    const int8_t *st = smem + buf * T::STAGE;
-    // All A fragments of the stage stay in registers; B fragments are read
-    // one slice at a time and used by every A slice it pairs with.
-    qs_v4i af[S][2];
-#pragma unroll
-    for (int p = 0; p < S; ++p)
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-        af[p][i] = *(const qs_v4i *)(st + ((p * 128) + wm * 32 + i * 16 + fr) * T::PITCH + 16 * fk);
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-      qs_v4i bf[2];
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-        bf[j] = *(const qs_v4i *)(st + ((q * 128) + 64 + wn * 32 + j * 16 + fr) * T::PITCH + 16 * fk);
-#pragma unroll
-      for (int p = 0; p + q < S; ++p)   // one-based p + q <= S + 1
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j)
-            acc[p + q][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[p][i], bf[j], acc[p + q][i][j], 0, 0, 0);
-    }
-    if (((kc + 1) & 127) == 0) flush();   // every 8192 K steps: INT32 never overflows
-    if (more) {
-      if (T::STAGES == 1) __syncthreads();   // everyone is done reading the only stage
-      int8_t *ws = smem + (T::STAGES == 2 ? (buf ^ 1) : 0) * T::STAGE;
-#pragma unroll
-      for (int i = 0; i < T::NW; ++i) *dst(ws, t + 256 * i) = pre[i];
-    }
-    __syncthreads();
-    if (T::STAGES == 2) buf ^= 1;
+template <bool MFAST>
+__global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const CrtConst c) {
+  __shared__ double tile[4][MFAST ? 64 : 16][MFAST ? 17 : 65];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t strip = (int64_t)blockIdx.x * 4 + wave;
+  int64_t mb0, nb0;
+  if (MFAST) {
+    mb0 = 4 * (strip % (a.MB / 4));
+    nb0 = strip / (a.MB / 4);
+  } else {
+    mb0 = strip % a.MB;
+    nb0 = 4 * (strip / a.MB);
   }
-  flush();
-
-  // Epilogue: undo the exact scales, then alpha / beta through C's tables.
   const GemmArgs &g = a.g;
+  const QS_GLOBAL unsigned int *u = (const QS_GLOBAL unsigned int *)a.u;
+  // Per block, all n residue words are loaded before any arithmetic.
+#pragma unroll 1
+  for (int b = 0; b < 4; ++b) {
+    const int64_t blk = (MFAST ? mb0 + b : mb0) * a.NB + (MFAST ? nb0 : nb0 + b);
+    const QS_GLOBAL unsigned int *up = u + blk * c.n * 64 + lane;
+    unsigned int w[kMaxModuli];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int l = 0; l < kMaxModuli; ++l) w[l] = l < c.n ? up[l * 64] : 0u;
+    double s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int l = 0; l < kMaxModuli; ++l) {
+      // Words past n are zero and their constants too: adding them is exact.
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double uv = (double)((w[l] >> (8 * r)) & 0xffu);
+        s1[r] = __builtin_fma(uv, c.q1[l], s1[r]);   // exact
+        s2[r] = __builtin_fma(uv, c.q2[l], s2[r]);
+      }
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int64_t row = m0 + wm * 32 + i * 16 + 4 * fk + r;
-      if (row >= g.M) continue;
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int64_t col = n0 + wn * 32 + j * 16 + fr;
-        if (col >= g.N) continue;
-        const double v = ldexp(accd[i][j][r], a.ea[row] + a.eb[col]);
+      const double q = rint((s1[r] + s2[r]) * c.invp);
+      const double cp = __builtin_fma(-q, c.p1, s1[r]) + __builtin_fma(-q, c.p2, s2[r]);
+      // Lane l holds row 4 (l / 16) + r, column l % 16 of the block.
+      const int lr = 4 * (lane >> 4) + r, lc = lane & 15;
+      if (MFAST) tile[wave][16 * b + lr][lc] = cp;
+      else tile[wave][lr][16 * b + lc] = cp;
+    }
+  }
+  __syncthreads();
+  if (MFAST) {
+    const int64_t row = mb0 * 16 + lane;
+    if (row < g.M) {
+      const RowScale rs = a.sa[row];
+      for (int cc = 0; cc < 16; ++cc) {
+        const int64_t col = nb0 * 16 + cc;
+        if (col >= g.N) break;
+        const RowScale cs = a.sb[col];
+        const double v = ldexp(tile[wave][lane][cc] * rs.iq * cs.iq, rs.sh + cs.sh);
         if (g.compute_type == QS_F64) store_c<double, false, true>(g, row, col, v, 0.0);
         else store_c<float, false, true>(g, row, col, (float)v, 0.0f);
       }
     }
+  } else {
+    const int64_t col = nb0 * 16 + lane;
+    if (col < g.N) {
+      const RowScale cs = a.sb[col];
+      for (int rr = 0; rr < 16; ++rr) {
+        const int64_t row = mb0 * 16 + rr;
+        if (row >= g.M) break;
+        const RowScale rs = a.sa[row];
+        const double v = ldexp(tile[wave][rr][lane] * rs.iq * cs.iq, rs.sh + cs.sh);
+        if (g.compute_type == QS_F64) store_c<double, false, true>(g, row, col, v, 0.0);
+        else store_c<float, false, true>(g, row, col, (float)v, 0.0f);
+      }
+    }
+  }
 }
 
 // Error report: for sampled (i, j), the FP64 result against a Dot2
@@ -341,146 +562,229 @@ __global__ __launch_bounds__(64) void split_report_kernel(GemmArgs g, const Spli
 
 // ------------------------------------------------------------------ host
 
-int split_slices_for(double target, int ct) {
+typedef unsigned __int128 u128;
+
+static int bitlen(u128 x) {
+  int b = 0;
+  while (x) { ++b; x >>= 1; }
+  return b;
+}
+
+static double u128_to_double(u128 x) {
+  return std::ldexp((double)(uint64_t)(x >> 64), 64) + (double)(uint64_t)x;
+}
+
+// Pairwise coprime moduli, largest first: 256, 255, 253, 251, 247, 241, ...
+const std::vector<int> &split_moduli_table() {
+  static const std::vector<int> tab = [] {
+    std::vector<int> v;
+    for (int m = 256; m >= 2 && (int)v.size() < kMaxModuli; --m) {
+      bool ok = true;
+      for (int x : v) {
+        int a = m, b = x;
+        while (b) { const int r = a % b; a = b; b = r; }
+        if (a != 1) { ok = false; break; }
+      }
+      if (ok) v.push_back(m);
+    }
+    return v;
+  }();
+  return tab;
+}
+
+static u128 moduli_product(int n) {
+  u128 p = 1;
+  for (int i = 0; i < n; ++i) p *= (u128)split_moduli_table()[i];
+  return p;
+}
+
+// Bound on the integer row norms: T = sqrt(P / 2) (1 - 2^-20) - sqrt(K) - 1,
+// so (T + sqrt(K) / 2)^2 stays below P / 2 by a margin the rounding of
+// rint(S / P) cannot cross.
+static double crt_T(int n, int64_t K) {
+  return std::sqrt(u128_to_double(moduli_product(n)) / 2.0) * (1.0 - 0x1p-20) - std::sqrt((double)K) - 1.0;
+}
+
+int split_moduli_for(double target, int ct, int64_t K) {
   if (!(target > 0)) target = ct == QS_F64 ? 1e-13 : 1e-6;
-  // Relative error of the truncated product about 2^(6 - 7S): the residual
-  // slices and the dropped pairs each contribute a few units of 2^-7S of the
-  // row / column maxima, and N(0,1) rows sit about 2 bits below their
-  // maximum (measured: 1.1e-13 at S = 7, M = N = K = 4096).
-  int s = (int)std::ceil((6.0 - std::log2(target)) / 7.0);
-  return std::max(2, std::min(8, s));
+  // Rounding A and B to integers of row norm T leaves a relative error of
+  // about 0.58 sqrt(K) / T in C (entries of random sign); a factor of two
+  // margin on top.
+  const double need = 1.16 * std::sqrt((double)std::max<int64_t>(K, 1)) / target;
+  for (int n = 2; n <= kMaxModuli; ++n)
+    if (crt_T(n, K) >= need) return n;
+  return kMaxModuli;
+}
+
+static CrtConst crt_constants(int n) {
+  CrtConst c{};
+  const std::vector<int> &mt = split_moduli_table();
+  const u128 P = moduli_product(n);
+  c.n = n;
+  u128 sum = 0;
+  std::vector<u128> Q(n);
+  for (int l = 0; l < n; ++l) {
+    Q[l] = P / (u128)mt[l];
+    sum += (u128)(mt[l] - 1) * Q[l];
+  }
+  const u128 top = std::max(sum, (u128)(n + 1) * P);
+  const int g = std::max(0, bitlen(top) - 53);
+  const u128 low = ((u128)1 << g) - 1;
+  for (int l = 0; l < n; ++l) {
+    const int m = mt[l];
+    c.mf[l] = (double)m;
+    c.invm[l] = 1.0 / (double)m;
+    c.cst[l] = 0.5 - 128.0 / (double)m;
+    const int qmod = (int)(Q[l] % (u128)m);
+    int y = 1;
+    while ((qmod * y) % m != 1) ++y;                   // (P/m)^-1 mod m
+    const int p32 = (int)(((u128)1 << 32) % (u128)m);   // 2^32 mod m
+    c.w[1][l][0] = (double)(1 % m);
+    c.w[1][l][1] = (double)p32;
+    c.w[0][l][0] = (double)(y % m);
+    c.w[0][l][1] = (double)((p32 * y) % m);
+    c.q1[l] = u128_to_double(Q[l] & ~low);              // exact: fewer than 53 significant bits
+    c.q2[l] = u128_to_double(Q[l] & low);
+  }
+  c.p1 = u128_to_double(P & ~low);
+  c.p2 = u128_to_double(P & low);
+  c.invp = 1.0 / u128_to_double(P);
+  return c;
 }
 
 static int64_t rup(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+struct SplitLayout {
+  size_t ra, rb, u, partA, partB, scA, scB, smp, rep, total;
+};
+
+static SplitLayout split_layout(const qs_plan *p) {
+  SplitLayout L;
+  const int n = p->moduli;
+  const int64_t nkc = (p->K + kNormChunk - 1) / kNormChunk;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = (off + bytes + 255) / 256 * 256;
+    return at;
+  };
+  L.ra = take((size_t)n * p->split_Mp * p->split_Kp);
+  L.rb = take((size_t)n * p->split_Np * p->split_Kp);
+  L.u = take((size_t)n * p->split_Mp * p->split_Np);
+  L.partA = take(sizeof(Ssq) * nkc * p->M);
+  L.partB = take(sizeof(Ssq) * nkc * p->N);
+  L.scA = take(sizeof(RowScale) * p->M);
+  L.scB = take(sizeof(RowScale) * p->N);
+  L.smp = take(sizeof(SplitSample) * kSplitSamples);
+  L.rep = take(sizeof(double) * 3 * kSplitSamples);
+  L.total = off;
+  return L;
+}
+
 qs_status split_prepare(qs_plan *p) {
-  const int64_t M = p->M, N = p->N, K = p->K;
-  p->split_Mp = rup(M, 64);
-  p->split_Np = rup(N, 64);
-  p->split_Kp = rup(std::max<int64_t>(K, 1), 64);
-  const int S = p->slices;
-  const size_t sa = (size_t)S * p->split_Mp * p->split_Kp, sb = (size_t)S * p->split_Np * p->split_Kp;
-  const size_t bytes = sa + sb + sizeof(unsigned long long) * (M + N) + sizeof(int) * (M + N) +
-                       sizeof(SplitSample) * kSplitSamples + sizeof(double) * 3 * kSplitSamples + 256;
-  if (hipMalloc(&p->d_split, bytes) != hipSuccess) return fail(QS_ERR_MEMORY, "plan_contract: split workspace");
-  // Padding rows and K stay zero for the plan's lifetime (written here once,
-  // outside any stream the caller may capture).
-  if (hipMemset(p->d_split, 0, sa + sb) != hipSuccess) return fail(QS_ERR_DEVICE, "plan_contract: split workspace");
-  p->split_bytes = (int64_t)bytes;
-  p->workspace_bytes += (int64_t)bytes;
+  p->split_Mp = rup(p->M, kTile);
+  p->split_Np = rup(p->N, kTile);
+  p->split_Kp = rup(std::max<int64_t>(p->K, 1), 128);   // an even number of 64-K stages
+  const SplitLayout L = split_layout(p);
+  if (hipMalloc(&p->d_split, L.total) != hipSuccess) return fail(QS_ERR_MEMORY, "plan_contract: split workspace");
+  p->split_bytes = (int64_t)L.total;
+  p->workspace_bytes += (int64_t)L.total;
   return QS_OK;
 }
 
-struct SplitPtrs {
-  int8_t *as, *bs;
-  unsigned long long *rmA, *rmB;
-  int *eA, *eB;
-  SplitSample *smp;
-  double *rep;
-};
-
-static SplitPtrs split_ptrs(qs_plan *p) {
-  SplitPtrs q;
-  char *c = (char *)p->d_split;
-  const size_t sa = (size_t)p->slices * p->split_Mp * p->split_Kp, sb = (size_t)p->slices * p->split_Np * p->split_Kp;
-  q.as = (int8_t *)c;
-  q.bs = (int8_t *)(c + sa);
-  size_t off = (sa + sb + 15) / 16 * 16;
-  q.rmA = (unsigned long long *)(c + off);
-  q.rmB = q.rmA + p->M;
-  off += sizeof(unsigned long long) * (p->M + p->N);
-  q.eA = (int *)(c + off);
-  q.eB = q.eA + p->M;
-  off += sizeof(int) * (p->M + p->N);
-  off = (off + 15) / 16 * 16;
-  q.smp = (SplitSample *)(c + off);
-  off += sizeof(SplitSample) * kSplitSamples;
-  q.rep = (double *)(c + off);
-  return q;
-}
-
-template <int S>
-static void launch_slicing(SplitOperand &o, hipStream_t s) {
-  dim3 grid((unsigned)((o.R + 63) / 64), (unsigned)((o.K + 63) / 64));
-  hipLaunchKernelGGL(split_rowmax_kernel, grid, dim3(256), 0, s, o);
-  hipLaunchKernelGGL(split_slice_kernel<S>, grid, dim3(256), 0, s, o);
-}
-
-template <int S>
-static hipError_t launch_split_s(qs_plan *p, const GemmArgs &g, const SplitPtrs &q, hipStream_t s) {
-  SplitOperand oa, ob;
-  oa.X = g.A; oa.type = g.a_type; oa.kfast = p->a_kfast; oa.base = g.a_base; oa.rowtab = g.mA; oa.ktab = g.kA;
-  oa.R = p->M; oa.K = p->K; oa.rowmax = q.rmA; oa.exps = q.eA; oa.slices = q.as; oa.Rp = p->split_Mp;
-  oa.Kp = p->split_Kp; oa.S = S;
-  ob.X = g.B; ob.type = g.b_type; ob.kfast = p->b_kfast; ob.base = g.b_base; ob.rowtab = g.nB; ob.ktab = g.kB;
-  ob.R = p->N; ob.K = p->K; ob.rowmax = q.rmB; ob.exps = q.eB; ob.slices = q.bs; ob.Rp = p->split_Np;
-  ob.Kp = p->split_Kp; ob.S = S;
-  const int64_t nmax = p->M + p->N;
-  hipLaunchKernelGGL(split_zero_kernel, dim3((unsigned)std::min<int64_t>((nmax + 255) / 256, 1024)), dim3(256), 0, s,
-                     q.rmA, nmax);
-  launch_slicing<S>(oa, s);
-  launch_slicing<S>(ob, s);
-  SplitGemmArgs a;
-  a.g = g;
-  a.as = q.as;
-  a.bs = q.bs;
-  a.ea = q.eA;
-  a.eb = q.eB;
-  a.Mp = p->split_Mp;
-  a.Np = p->split_Np;
-  a.Kp = p->split_Kp;
-  const int64_t tiles = (a.Mp / 64) * (a.Np / 64);
-  hipLaunchKernelGGL(split_gemm_kernel<S>, dim3((unsigned)tiles), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
 hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t s) {
-  const SplitPtrs q = split_ptrs(p);
-  int n = 0;
+  const SplitLayout L = split_layout(p);
+  char *base = (char *)p->d_split;
+  SplitSample *smp = (SplitSample *)(base + L.smp);
+  double *rep = (double *)(base + L.rep);
+  int nsmp = 0;
   if (report) {
-    n = (int)std::min<int64_t>(kSplitSamples, p->M * p->N);
+    nsmp = (int)std::min<int64_t>(kSplitSamples, p->M * p->N);
     // Deterministic sample positions: a fixed stride walk over the M x N grid.
-    std::vector<SplitSample> h(n);
+    std::vector<SplitSample> h(nsmp);
     const int64_t tot = p->M * p->N;
-    for (int i = 0; i < n; ++i) {
-      const int64_t e = (int64_t)((double)i * (double)tot / (double)n + 0.5) % tot;
+    for (int i = 0; i < nsmp; ++i) {
+      const int64_t e = (int64_t)((double)i * (double)tot / (double)nsmp + 0.5) % tot;
       h[i].i = e % p->M;
       h[i].j = e / p->M;
       h[i].c_old = 0;
     }
-    hipError_t e = hipMemcpyAsync(q.smp, h.data(), sizeof(SplitSample) * n, hipMemcpyHostToDevice, s);
+    hipError_t e = hipMemcpyAsync(smp, h.data(), sizeof(SplitSample) * nsmp, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(split_sample_old_kernel, dim3((n + 63) / 64), dim3(64), 0, s, g, q.smp, n);
+    hipLaunchKernelGGL(split_sample_old_kernel, dim3((nsmp + 63) / 64), dim3(64), 0, s, g, smp, nsmp);
     e = hipStreamSynchronize(s);   // `h` goes out of scope
     if (e != hipSuccess) return e;
   }
-  hipError_t e;
-  switch (p->slices) {
-    case 2: e = launch_split_s<2>(p, g, q, s); break;
-    case 3: e = launch_split_s<3>(p, g, q, s); break;
-    case 4: e = launch_split_s<4>(p, g, q, s); break;
-    case 5: e = launch_split_s<5>(p, g, q, s); break;
-    case 6: e = launch_split_s<6>(p, g, q, s); break;
-    case 7: e = launch_split_s<7>(p, g, q, s); break;
-    default: e = launch_split_s<8>(p, g, q, s); break;
+
+  const int n = p->moduli;
+  const CrtConst c = crt_constants(n);
+  int eT;
+  const double Tm = std::frexp(crt_T(n, p->K), &eT);
+  const int nkc = (int)((p->K + kNormChunk - 1) / kNormChunk);
+  CrtOperand oa;
+  oa.X = g.A; oa.type = g.a_type; oa.kfast = p->a_kfast; oa.base = g.a_base; oa.rowtab = g.mA; oa.ktab = g.kA;
+  oa.R = p->M; oa.K = p->K; oa.Rp = p->split_Mp; oa.Kp = p->split_Kp; oa.nkc = nkc; oa.side = 0;
+  oa.part = (Ssq *)(base + L.partA); oa.scale = (RowScale *)(base + L.scA); oa.res = (int8_t *)(base + L.ra);
+  oa.Tm = Tm; oa.eT = eT;
+  CrtOperand ob = oa;
+  ob.X = g.B; ob.type = g.b_type; ob.kfast = p->b_kfast; ob.base = g.b_base; ob.rowtab = g.nB; ob.ktab = g.kB;
+  ob.R = p->N; ob.Rp = p->split_Np; ob.side = 1;
+  ob.part = (Ssq *)(base + L.partB); ob.scale = (RowScale *)(base + L.scB); ob.res = (int8_t *)(base + L.rb);
+  for (const CrtOperand *o : {&oa, &ob}) {
+    hipLaunchKernelGGL(crt_norm_kernel, dim3((unsigned)((o->R + 63) / 64), (unsigned)nkc), dim3(256), 0, s, *o);
+    hipLaunchKernelGGL(crt_residue_kernel, dim3((unsigned)(o->Rp / 64), (unsigned)(o->Kp / 64)), dim3(256), 0, s,
+                       *o, c);
   }
+
+  CrtGemmArgs ga;
+  ga.ra = oa.res;
+  ga.rb = ob.res;
+  ga.u = (unsigned int *)(base + L.u);
+  ga.MB = p->split_Mp / 16;
+  ga.NB = p->split_Np / 16;
+  ga.KB = p->split_Kp / 64;
+  ga.n = n;
+  for (int l = 0; l < kMaxModuli; ++l) {
+    ga.m[l] = l < n ? split_moduli_table()[l] : 1;
+    ga.invm[l] = 1.0f / (float)ga.m[l];
+  }
+  // 256 x 256 tiles, 8 waves of 128 x 64, one workgroup per CU: measured
+  // ahead of 128 x 128 (4 waves, 3 per CU) and 256 x 128 (8 waves, 2 per CU).
+  ga.tiles_m = p->split_Mp / kTile;
+  ga.tiles_n = p->split_Np / kTile;
+  hipLaunchKernelGGL((crt_gemm_kernel<2, 4, 8, 4, 1>), dim3((unsigned)(ga.tiles_m * ga.tiles_n * n)), dim3(512), 0, s,
+                     ga);
+
+  CrtRecArgs ra;
+  ra.g = g;
+  ra.u = ga.u;
+  ra.sa = oa.scale;
+  ra.sb = ob.scale;
+  ra.MB = ga.MB;
+  ra.NB = ga.NB;
+  // Strips of four blocks, four per workgroup: Mp and Np are multiples of 256.
+  const dim3 rgrid((unsigned)(ga.MB * ga.NB / 16));
+  if (p->c_mfast) hipLaunchKernelGGL(crt_rec_kernel<true>, rgrid, dim3(256), 0, s, ra, c);
+  else hipLaunchKernelGGL(crt_rec_kernel<false>, rgrid, dim3(256), 0, s, ra, c);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess || !report) return e;
-  hipLaunchKernelGGL(split_report_kernel, dim3(n), dim3(64), 0, s, g, q.smp, q.rep);
+
+  hipLaunchKernelGGL(split_report_kernel, dim3(nsmp), dim3(64), 0, s, g, smp, rep);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  std::vector<double> out(3 * n);
-  e = hipMemcpyAsync(out.data(), q.rep, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s);
+  std::vector<double> out(3 * nsmp);
+  e = hipMemcpyAsync(out.data(), rep, sizeof(double) * 3 * nsmp, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return e;
   double num = 0, den = 0, mx = 0;
-  for (int i = 0; i < n; ++i) {
+  for (int i = 0; i < nsmp; ++i) {
     const double ref = out[3 * i + 1] + out[3 * i + 2];
     const double d = std::fabs((out[3 * i] - out[3 * i + 1]) - out[3 * i + 2]);
     num += d * d;
     den += ref * ref;
     if (ref != 0) mx = std::max(mx, d / std::fabs(ref));
   }
-  p->split_report.samples = n;
+  p->split_report.samples = nsmp;
   p->split_report.rel_fro = den > 0 ? std::sqrt(num / den) : std::sqrt(num);
   p->split_report.max_rel = mx;
   return hipSuccess;

@@ -1,4 +1,4 @@
-"""Split precision (Ozaki scheme on INT8 MFMA) against the oracle, with the
+"""Split precision (Ozaki scheme II: CRT over INT8 MFMA GEMMs) against the oracle, with the
 delivered-error report.  Tolerances: relative Frobenius <= 1e-13 (FP64
 target) and <= 1e-5 (FP32)."""
 import numpy as np
@@ -43,7 +43,7 @@ def test_split_f64_rank4_meets_1e13(oracle, d, beta):
     Cv, Av, Bv = rank4(rng, d)
     c0, a0, b0 = storage_np(Cv.parent), storage_np(Av.parent), storage_np(Bv.parent)
     plan = qs.plan_contract(Cv, Av, (1, 5, 2, 6), Bv, (5, 3, 6, 4), (1, 2, 3, 4), precision="split")
-    assert plan.path == "split" and plan.info["slices"] >= 6, plan.info
+    assert plan.path == "split" and plan.info["moduli"] >= 10, plan.info
     qs.set_error_report(plan, True)
     qs.execute(plan, 1.25, beta)
     torch.cuda.synchronize()
@@ -51,7 +51,7 @@ def test_split_f64_rank4_meets_1e13(oracle, d, beta):
     want = oracle_of_plan(oracle, plan, 1.25, beta, a0, b0, c0)
     err = rel_fro(got, want)
     rep = qs.error_report(plan)
-    print(f"[split] d={d} slices={plan.info['slices']} err={err:.3e} report={rep}", flush=True)
+    print(f"[split] d={d} moduli={plan.info['moduli']} err={err:.3e} report={rep}", flush=True)
     assert err <= 1e-13, err
     assert rep["samples"] > 0 and rep["rel_fro"] <= 1e-13, rep
 
@@ -108,3 +108,81 @@ def test_split_complex_is_rejected():
     Cv = SV(torch.zeros(16, dtype=torch.complex128, device=DEV), (4, 4), (1, 4))
     with pytest.raises(qs.UnsupportedError):
         qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split")
+
+
+def _plain_pair(rng, M, K, N, row_exp=None):
+    A = rng.standard_normal((M, K))
+    if row_exp is not None:
+        A *= np.exp2(row_exp)[:, None]
+    B = rng.standard_normal((K, N))
+    Av = SV(torch.from_numpy(A.reshape(-1, order="F").copy()).to(DEV), (M, K), (1, M))
+    Bv = SV(torch.from_numpy(B.reshape(-1, order="F").copy()).to(DEV), (K, N), (1, K))
+    Cv = SV(torch.from_numpy(rng.standard_normal(M * N)).to(DEV), (M, N), (1, M))
+    return A, B, Av, Bv, Cv
+
+
+def test_split_long_k_reduces_accumulators(oracle):
+    case("split F64, K = 9000 (> 4096: INT32 accumulators reduced mod m mid-K)")
+    rng = np.random.default_rng(11)
+    M, K, N = 70, 9000, 40
+    A, B, Av, Bv, Cv = _plain_pair(rng, M, K, N)
+    c0 = storage_np(Cv.parent)
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split")
+    qs.execute(plan, 1.0, 0.25)
+    torch.cuda.synchronize()
+    want = oracle_of_plan(oracle, plan, 1.0, 0.25, storage_np(Av.parent), storage_np(Bv.parent), c0)
+    assert rel_fro(storage_np(Cv.parent), want) <= 1e-13
+
+
+def test_split_extreme_row_scales_rowwise(oracle):
+    case("split F64, rows of A scaled by 2^e, e in [-900, 900]: per-row error")
+    rng = np.random.default_rng(12)
+    M, K, N = 64, 200, 48
+    e = rng.integers(-900, 901, size=M)
+    A, B, Av, Bv, Cv = _plain_pair(rng, M, K, N, row_exp=e)
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split")
+    qs.execute(plan, 1.0, 0.0)
+    torch.cuda.synchronize()
+    got = storage_np(Cv.parent).reshape(N, M).T
+    want = oracle_of_plan(oracle, plan, 1.0, 0.0, storage_np(Av.parent), storage_np(Bv.parent),
+                          np.zeros(M * N)).reshape(N, M).T
+    for i in range(M):
+        sc = np.abs(want[i]).max()   # norms of the rescaled rows: no overflow at 2^900
+        err = np.linalg.norm((got[i] - want[i]) / sc) / np.linalg.norm(want[i] / sc)
+        assert err <= 1e-13, (i, e[i], err)
+
+
+def test_split_nan_row_and_zero_row(oracle):
+    case("split F64: a NaN in row 3 of A poisons row 3 of C only; an all-zero row 7 gives beta * C")
+    rng = np.random.default_rng(13)
+    M, K, N = 40, 64, 24
+    A, B, Av, Bv, Cv = _plain_pair(rng, M, K, N)
+    A[3, 5] = np.nan
+    A[7, :] = 0.0
+    Av = SV(torch.from_numpy(A.reshape(-1, order="F").copy()).to(DEV), (M, K), (1, M))
+    c0 = storage_np(Cv.parent).copy()
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split")
+    qs.execute(plan, 1.0, 0.5)
+    torch.cuda.synchronize()
+    got = storage_np(Cv.parent).reshape(N, M).T
+    assert np.isnan(got[3]).all()
+    assert np.array_equal(got[7], 0.5 * c0.reshape(N, M).T[7])
+    ok = np.ones(M, bool)
+    ok[3] = False
+    want = oracle_of_plan(oracle, plan, 1.0, 0.5, np.nan_to_num(storage_np(Av.parent)), storage_np(Bv.parent),
+                          c0).reshape(N, M).T
+    assert rel_fro(got[ok], want[ok]) <= 1e-13
+
+
+def test_split_is_bitwise_reproducible():
+    case("split F64: two executes with beta = 0 give identical bits")
+    rng = np.random.default_rng(14)
+    M, K, N = 300, 500, 260
+    A, B, Av, Bv, Cv = _plain_pair(rng, M, K, N)
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split")
+    qs.execute(plan, 1.0, 0.0)
+    torch.cuda.synchronize()
+    first = storage_np(Cv.parent).copy()
+    qs.execute(plan, 1.0, 0.0)
+    torch.cuda.synchronize()
+    assert np.array_equal(storage_np(Cv.parent), first)
