@@ -362,6 +362,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotune cache file: loaded if it exists (no tuning runs), else written after tuning")
+    ap.add_argument("--only", default=None,
+                    choices=["split", "grouped_f64", "grouped_c64", "served"],
+                    help="only run one of the extra workloads and print its JSON (for profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 512 MiB device-to-device copy 5 times (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -373,12 +380,50 @@ def main():
     qs = _load()
     dev = torch.device("cuda:0")
 
+    if args.copy_calibration:
+        n = (512 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read_per_copy": 8 * n, "bytes_written_per_copy": 8 * n,
+                                               "copies": 5}}), flush=True)
+        return
+
+    if args.only is not None:
+        if args.only == "split":
+            out = bench_split(qs, torch, dev)
+        elif args.only.startswith("grouped"):
+            out = bench_grouped(qs, torch, dev, args.only.endswith("c64"))
+        else:
+            pools = served_pools(qs, torch, dev)
+            out = {"grouped": bench_served(qs, torch, dev, pools, True),
+                   "per_request": bench_served(qs, torch, dev, pools, False)}
+        print(json.dumps({args.only: out}), flush=True)
+        return
+
+    cached = args.tune_cache is not None and os.path.exists(args.tune_cache)
+    if cached:
+        qs.autotune_load(args.tune_cache)
     plan, flops, bytes_, views, beta = config2(qs, torch, np.float64, dev)
     c_init = views[3].parent.cpu().numpy().copy()
-    tune = qs.autotune(plan, reps=10)
+    if cached:
+        tune = {"from_cache": args.tune_cache, "tile_m": plan.info["tile_m"], "tile_n": plan.info["tile_n"],
+                "k_splits": plan.info["k_splits"]}
+    else:
+        tune = qs.autotune(plan, reps=10)
+        if args.tune_cache:
+            qs.autotune_save(args.tune_cache)
     wall, kern_mean, kern_med = time_plan(qs, torch, plan, beta, args.steps, args.warmup)
     value = flops * args.steps / wall / 1e9
     achieved = flops / kern_mean
+    # HBM bytes per launch of the GEMM kernel, from the committed FETCH_SIZE /
+    # WRITE_SIZE counter runs (scaled by a plain copy's reading).
+    traffic_bytes, traffic_src = None, os.path.join(ROOT, "profiles", "headline_traffic.json")
+    if os.path.exists(traffic_src):
+        with open(traffic_src) as f:
+            traffic_bytes = json.load(f).get("traffic_bytes_per_launch")
 
     result = {
         "metric": "GFLOP/s (plain FP64 block-scatter contraction)",
@@ -406,7 +451,7 @@ def main():
             "peak": 78.6,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64, 4),
-            "traffic": None,
+            "traffic": traffic_bytes,
             "kernel_ms_mean": round(kern_mean * 1e3, 5),
             "kernel_ms_median": round(kern_med * 1e3, 5),
         },

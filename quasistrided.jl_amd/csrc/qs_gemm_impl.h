@@ -8,6 +8,8 @@
 //   * one workgroup (4 waves, 2x2) per BM x BN tile of C, full K (or one K
 //     split) accumulated in MFMA registers -- beta applied once, in the
 //     epilogue, by construction (nest.jl:83 applies it on the first K block);
+//     with split-K the last split of a tile to finish adds the partials in a
+//     fixed order and runs the epilogue;
 //   * the pack is a gather fused into the global->LDS load: address =
 //     base + rowoffset[i] + koffset[k] (Tile, src/layout/tiles.jl:102-103),
 //     lanes laid along whichever of the free or the K group is unit-stride in
@@ -20,6 +22,8 @@
 //     C's storage type and scatter-stored through C's offset tables; beta == 0
 //     never reads C.
 #pragma once
+
+#include <type_traits>
 
 #include "qs_internal.h"
 
@@ -106,7 +110,44 @@ __device__ __forceinline__ void store_elem(void *p, int t, int64_t idx, R re, R 
 
 // The epilogue at one element of C: `alpha*x + beta*C` for the beta case,
 // with Base's Complex expression trees (stores.jl:91-124; the library is
-// built with -ffp-contract=off so the unfused products stay unfused).
+// built with -ffp-contract=off so the unfused products stay unfused).  Split
+// in two so callers can issue every old-C load of a batch before the first
+// store (loads after a store to C cannot be hoisted above it).
+template <typename R, bool CPLX, bool MIXED>
+__device__ __forceinline__ void load_c_old(const GemmArgs &g, int64_t idx, R &o, R &oi) {
+  o = 0;
+  oi = 0;
+  if (g.beta_case != 0) load_elem<R, CPLX, MIXED>(g.C, g.c_type, idx, o, oi);
+}
+
+template <typename R, bool CPLX, bool MIXED>
+__device__ __forceinline__ void store_c_at(const GemmArgs &g, int64_t idx, R xr, R xi, R o, R oi) {
+  const R ar = (R)g.alpha_re, ai = (R)g.alpha_im;
+  R outr, outi = 0;
+  if (!CPLX) {
+    if (g.beta_case == 0) outr = ar * xr;
+    else outr = g.beta_case == 1 ? qs_fma(ar, xr, o) : qs_fma(ar, xr, (R)g.beta_re * o);
+  } else {
+    if (g.beta_case == 0) {
+      R t1 = ar * xr, t2 = ai * xi, t3 = ar * xi, t4 = ai * xr;
+      outr = t1 - t2;
+      outi = t3 + t4;
+    } else {
+      R tr = o, ti = oi;
+      if (g.beta_case == 2) {
+        const R br = (R)g.beta_re, bi = (R)g.beta_im;
+        R u1 = br * o, u2 = bi * oi, u3 = br * oi, u4 = bi * o;
+        tr = u1 - u2;
+        ti = u3 + u4;
+      }
+      outr = qs_fma(ar, xr, -qs_fma(ai, xi, -tr));
+      outi = qs_fma(ar, xi, qs_fma(ai, xr, ti));
+    }
+  }
+  store_elem<R, CPLX, MIXED>(g.C, g.c_type, idx, outr, outi);
+}
+
+// Both halves at one element (old C loaded only when beta != 0).
 template <typename R, bool CPLX, bool MIXED>
 __device__ __forceinline__ void store_c(const GemmArgs &g, int64_t row, int64_t col, R xr, R xi) {
   const int64_t idx = g.c_base + g.mC[row] + g.nC[col];
@@ -208,7 +249,7 @@ struct Gather {
   int row0, kk0;
   int64_t koff[NK];
   bool kok[NK];
-  R val[NL][P];
+  R val[2][NL][P];   // two stages in flight
 
   __device__ __forceinline__ void init(int t, int64_t r0, int64_t rows_total, int64_t base,
                                        const int64_t *__restrict__ rowtab) {
@@ -234,6 +275,7 @@ struct Gather {
   // Issue the stage's loads.  An element outside the tile reads the device
   // zero instead (a select on the address, not on the value), so nothing
   // waits on the loads until `stage` consumes them.
+  template <int S>
   __device__ __forceinline__ void load(const void *__restrict__ p, int type) {
 #pragma unroll
     for (int r = 0; r < NL; ++r) {
@@ -242,11 +284,11 @@ struct Gather {
       const void *src = ok ? p : (const void *)qs_zero16;
       const int64_t idx = ok ? rowoff[ir] + koff[kr] : 0;
       R im;
-      load_elem<R, CPLX, MIXED>(src, type, idx, val[r][0], im);
-      if (CPLX) val[r][P - 1] = im;
+      load_elem<R, CPLX, MIXED>(src, type, idx, val[S][r][0], im);
+      if (CPLX) val[S][r][P - 1] = im;
     }
   }
-  template <int PLANE>
+  template <int S, int PLANE>
   __device__ __forceinline__ void stage(R *lds, int t, R sgn) {
     typedef LdsTile<R, ROWS, BK, KF> L;
 #pragma unroll
@@ -254,8 +296,8 @@ struct Gather {
       if (!PARTIAL || t + 256 * r < ROWS * BK) {
         const int row = row0 + (KF ? r * RSTEP : 0);
         const int k = kk0 + (KF ? 0 : r * KSTEP);
-        lds[L::at(row, k)] = val[r][0];
-        if (CPLX) lds[PLANE + L::at(row, k)] = sgn * val[r][P - 1];
+        lds[L::at(row, k)] = val[S][r][0];
+        if (CPLX) lds[PLANE + L::at(row, k)] = sgn * val[S][r][P - 1];
       }
     }
   }
@@ -277,7 +319,12 @@ struct TileGemm {
   static_assert(256 % BM == 0 && 256 % BN == 0, "tile rows");
   static_assert(BK % 4 == 0, "BK");
 
+  // DEEPEN: keep two stages of gathered operands in flight where registers
+  // allow (real, one storage type, tiles up to 64 x 128); the grouped kernel,
+  // whose tile classes share one register allocation, passes false.
+  template <bool DEEPEN = true>
   __device__ static void run(const GemmArgs &g, int64_t tm, int64_t tn, int split, R *smem) {
+    constexpr bool DEEP = DEEPEN && !CPLX && !MIXED && BM * BN <= 64 * 128;
     typedef typename Acc4<R>::type V;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
@@ -301,123 +348,271 @@ struct TileGemm {
         for (int b = 0; b < FN; ++b) acc[p][a][b] = V{0, 0, 0, 0};
 
     const int fr = lane & 15, fk = lane >> 4;
-    int buf = 0;
-    if (kbeg < kend) {
-      ga.fetch_k(kbeg, kend, g.kA);
-      gb.fetch_k(kbeg, kend, g.kB);
-      ga.load(g.A, g.a_type);
-      gb.load(g.B, g.b_type);
-      if (kbeg + BK < kend) {
+    if constexpr (DEEP) {
+      // Stage j of the K range lives in LDS buffer j & 1 and, while in flight,
+      // in gather register set j & 1.  Iteration j issues the loads of stage
+      // j + 2 (unconditionally: past kend every lane reads the device zero, so
+      // the compiler's wait counts stay exact), multiplies stage j and stores
+      // stage j + 1 into LDS: each stage's loads have two iterations to land.
+      auto step = [&](int64_t k0, auto cur, auto nxt) {
+        constexpr int C = decltype(cur)::value, N = decltype(nxt)::value;
+        ga.template load<C>(g.A, g.a_type);
+        gb.template load<C>(g.B, g.b_type);
+        ga.fetch_k(k0 + 3 * BK, kend, g.kA);
+        gb.fetch_k(k0 + 3 * BK, kend, g.kB);
+        __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the MFMAs
+        const R *As = smem + C * (SA + SB);
+        const R *Bs = As + SA;
+  #pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+          R af[P][FM], bf[P][FN];
+  #pragma unroll
+          for (int p = 0; p < P; ++p) {
+  #pragma unroll
+            for (int a = 0; a < FM; ++a) af[p][a] = As[p * LA::plane + LA::at(wm * WTM + a * 16 + fr, ks * 4 + fk)];
+  #pragma unroll
+            for (int b = 0; b < FN; ++b) bf[p][b] = Bs[p * LB::plane + LB::at(wn * WTN + b * 16 + fr, ks * 4 + fk)];
+          }
+  #pragma unroll
+          for (int a = 0; a < FM; ++a)
+  #pragma unroll
+            for (int b = 0; b < FN; ++b) {
+              if (!CPLX) {
+                acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+              } else {
+                acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+                acc[0][a][b] = mfma16(-af[P - 1][a], bf[P - 1][b], acc[0][a][b]);
+                acc[P - 1][a][b] = mfma16(af[0][a], bf[P - 1][b], acc[P - 1][a][b]);
+                acc[P - 1][a][b] = mfma16(af[P - 1][a], bf[0][b], acc[P - 1][a][b]);
+              }
+            }
+        }
+        R *Ws = smem + N * (SA + SB);
+        ga.template stage<N, LA::plane>(Ws, t, sa);
+        gb.template stage<N, LB::plane>(Ws + SA, t, sb);
+        __syncthreads();
+      };
+      if (kbeg < kend) {
+        ga.fetch_k(kbeg, kend, g.kA);
+        gb.fetch_k(kbeg, kend, g.kB);
+        ga.template load<0>(g.A, g.a_type);
+        gb.template load<0>(g.B, g.b_type);
         ga.fetch_k(kbeg + BK, kend, g.kA);
         gb.fetch_k(kbeg + BK, kend, g.kB);
-      }
-      ga.template stage<LA::plane>(smem, t, sa);
-      gb.template stage<LB::plane>(smem + SA, t, sb);
-    }
-    __syncthreads();
-    for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
-      const bool more = k0 + BK < kend;
-      if (more) {
-        ga.load(g.A, g.a_type);
-        gb.load(g.B, g.b_type);
-        if (k0 + 2 * BK < kend) {
-          ga.fetch_k(k0 + 2 * BK, kend, g.kA);
-          gb.fetch_k(k0 + 2 * BK, kend, g.kB);
-        }
-      }
-      const R *As = smem + buf * (SA + SB);
-      const R *Bs = As + SA;
-#pragma unroll
-      for (int ks = 0; ks < BK / 4; ++ks) {
-        R af[P][FM], bf[P][FN];
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-#pragma unroll
-          for (int a = 0; a < FM; ++a) af[p][a] = As[p * LA::plane + LA::at(wm * WTM + a * 16 + fr, ks * 4 + fk)];
-#pragma unroll
-          for (int b = 0; b < FN; ++b) bf[p][b] = Bs[p * LB::plane + LB::at(wn * WTN + b * 16 + fr, ks * 4 + fk)];
-        }
-#pragma unroll
-        for (int a = 0; a < FM; ++a)
-#pragma unroll
-          for (int b = 0; b < FN; ++b) {
-            if (!CPLX) {
-              acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
-            } else {
-              acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
-              acc[0][a][b] = mfma16(-af[P - 1][a], bf[P - 1][b], acc[0][a][b]);
-              acc[P - 1][a][b] = mfma16(af[0][a], bf[P - 1][b], acc[P - 1][a][b]);
-              acc[P - 1][a][b] = mfma16(af[P - 1][a], bf[0][b], acc[P - 1][a][b]);
-            }
-          }
-      }
-      if (more) {
-        R *Ws = smem + (buf ^ 1) * (SA + SB);
-        ga.template stage<LA::plane>(Ws, t, sa);
-        gb.template stage<LB::plane>(Ws + SA, t, sb);
+        ga.template load<1>(g.A, g.a_type);
+        gb.template load<1>(g.B, g.b_type);
+        ga.fetch_k(kbeg + 2 * BK, kend, g.kA);
+        gb.fetch_k(kbeg + 2 * BK, kend, g.kB);
+        ga.template stage<0, LA::plane>(smem, t, sa);
+        gb.template stage<0, LB::plane>(smem + SA, t, sb);
       }
       __syncthreads();
-      buf ^= 1;
+      typedef std::integral_constant<int, 0> S0;
+      typedef std::integral_constant<int, 1> S1;
+      // Two stages per trip, both unconditional: with an odd stage count the
+      // last one is all zeros (every lane past kend reads the device zero).
+      for (int64_t k0 = kbeg; k0 < kend; k0 += 2 * BK) {
+        step(k0, S0(), S1());
+        step(k0 + BK, S1(), S0());
+      }
+    } else {
+      // One stage in flight: stage j + 1 is loaded while stage j multiplies.
+      int buf = 0;
+      if (kbeg < kend) {
+        ga.fetch_k(kbeg, kend, g.kA);
+        gb.fetch_k(kbeg, kend, g.kB);
+        ga.template load<0>(g.A, g.a_type);
+        gb.template load<0>(g.B, g.b_type);
+        if (kbeg + BK < kend) {
+          ga.fetch_k(kbeg + BK, kend, g.kA);
+          gb.fetch_k(kbeg + BK, kend, g.kB);
+        }
+        ga.template stage<0, LA::plane>(smem, t, sa);
+        gb.template stage<0, LB::plane>(smem + SA, t, sb);
+      }
+      __syncthreads();
+      for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+        const bool more = k0 + BK < kend;
+        if (more) {
+          ga.template load<0>(g.A, g.a_type);
+          gb.template load<0>(g.B, g.b_type);
+          if (k0 + 2 * BK < kend) {
+            ga.fetch_k(k0 + 2 * BK, kend, g.kA);
+            gb.fetch_k(k0 + 2 * BK, kend, g.kB);
+          }
+        }
+        const R *As = smem + buf * (SA + SB);
+        const R *Bs = As + SA;
+  #pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+          R af[P][FM], bf[P][FN];
+  #pragma unroll
+          for (int p = 0; p < P; ++p) {
+  #pragma unroll
+            for (int a = 0; a < FM; ++a) af[p][a] = As[p * LA::plane + LA::at(wm * WTM + a * 16 + fr, ks * 4 + fk)];
+  #pragma unroll
+            for (int b = 0; b < FN; ++b) bf[p][b] = Bs[p * LB::plane + LB::at(wn * WTN + b * 16 + fr, ks * 4 + fk)];
+          }
+  #pragma unroll
+          for (int a = 0; a < FM; ++a)
+  #pragma unroll
+            for (int b = 0; b < FN; ++b) {
+              if (!CPLX) {
+                acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+              } else {
+                acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+                acc[0][a][b] = mfma16(-af[P - 1][a], bf[P - 1][b], acc[0][a][b]);
+                acc[P - 1][a][b] = mfma16(af[0][a], bf[P - 1][b], acc[P - 1][a][b]);
+                acc[P - 1][a][b] = mfma16(af[P - 1][a], bf[0][b], acc[P - 1][a][b]);
+              }
+            }
+        }
+        if (more) {
+          R *Ws = smem + (buf ^ 1) * (SA + SB);
+          ga.template stage<0, LA::plane>(Ws, t, sa);
+          gb.template stage<0, LB::plane>(Ws + SA, t, sb);
+        }
+        __syncthreads();
+        buf ^= 1;
+      }
     }
 
-    // Epilogue.
+    // Epilogue.  Complex, mixed-type and 128 x 128 accumulators leave no
+    // registers to spare: one element at a time.  Otherwise every old-C load
+    // of a 16-row block row is issued before its first store.
+    constexpr bool HEAVY = CPLX || MIXED || FM * FN * 4 * (int)sizeof(R) / 4 >= 128;
+    if (HEAVY || g.k_splits > 1 || g.beta_case == 0) {
 #pragma unroll
-    for (int a = 0; a < FM; ++a)
+      for (int a = 0; a < FM; ++a)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm * WTM + a * 16 + mfma_row<R>(fk, r);
-        if (row >= g.M) continue;
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = m0 + wm * WTM + a * 16 + mfma_row<R>(fk, r);
+          if (row >= g.M) continue;
+#pragma unroll
+          for (int b = 0; b < FN; ++b) {
+            const int64_t col = n0 + wn * WTN + b * 16 + fr;
+            if (col >= g.N) continue;
+            const R xr = acc[0][a][b][r];
+            const R xi = CPLX ? acc[P - 1][a][b][r] : (R)0;
+            if (g.k_splits == 1) {
+              store_c<R, CPLX, MIXED>(g, row, col, xr, xi);
+            } else {
+              R *part = (R *)g.partials;
+              const int64_t plane = g.M * g.N;
+              part[((int64_t)split * P) * plane + row * g.N + col] = xr;
+              if (CPLX) part[((int64_t)split * P + 1) * plane + row * g.N + col] = xi;
+            }
+          }
+        }
+    } else {
+#pragma unroll
+      for (int a = 0; a < FM; ++a) {
+        int64_t moff[4], noff[FN];
+        bool rok[4], cok[FN];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = m0 + wm * WTM + a * 16 + mfma_row<R>(fk, r);
+          rok[r] = row < g.M;
+          moff[r] = rok[r] ? g.mC[row] : 0;
+        }
 #pragma unroll
         for (int b = 0; b < FN; ++b) {
           const int64_t col = n0 + wn * WTN + b * 16 + fr;
-          if (col >= g.N) continue;
-          const R xr = acc[0][a][b][r];
-          const R xi = CPLX ? acc[P - 1][a][b][r] : (R)0;
-          if (g.k_splits == 1) {
-            store_c<R, CPLX, MIXED>(g, row, col, xr, xi);
-          } else {
-            R *part = (R *)g.partials;
-            const int64_t plane = g.M * g.N;
-            part[((int64_t)split * P) * plane + row * g.N + col] = xr;
-            if (CPLX) part[((int64_t)split * P + 1) * plane + row * g.N + col] = xi;
-          }
+          cok[b] = col < g.N;
+          noff[b] = cok[b] ? g.nC[col] : 0;
         }
+        R o[4][FN], oi[4][FN];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) {
+            o[r][b] = 0;
+            oi[r][b] = 0;
+            if (rok[r] && cok[b]) load_c_old<R, CPLX, MIXED>(g, g.c_base + moff[r] + noff[b], o[r][b], oi[r][b]);
+          }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int b = 0; b < FN; ++b)
+            if (rok[r] && cok[b])
+              store_c_at<R, CPLX, MIXED>(g, g.c_base + moff[r] + noff[b], acc[0][a][b][r], (R)0, o[r][b], oi[r][b]);
       }
+    }
   }
 };
+
+// Split-K: every split of a tile stores its partial sums, then counts itself
+// in on the tile's counter (release); the last to arrive (acquire) adds the
+// partials in split order 0, 1, ..., S-1 -- a fixed order, so results do not
+// depend on scheduling -- applies the epilogue and resets the counter.  No
+// workgroup waits for another.
+template <typename R, bool CPLX, bool MIXED, int BM, int BN>
+__device__ __forceinline__ void splitk_fixup(const GemmArgs &g, int64_t tile, int64_t tm, int64_t tn, int *last) {
+  constexpr int P = CPLX ? 2 : 1;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) *last = atomicAdd(&g.tile_counters[tile], 1) == g.k_splits - 1;
+  __syncthreads();
+  if (!*last) return;
+  __threadfence();
+  const int64_t plane = g.M * g.N;
+  const QS_GLOBAL R *part = (const QS_GLOBAL R *)g.partials;
+  // Batches of 8 elements per thread: every partial and old-C load of a
+  // batch is issued before its sums and stores.
+  constexpr int EPT = BM * BN / 256, CW = (CPLX || MIXED || BM * BN >= 128 * 128) ? 2 : 8, CH = EPT < CW ? EPT : CW;
+#pragma unroll 1
+  for (int c0 = 0; c0 < EPT; c0 += CH) {
+    int64_t idx[CH], cidx[CH];
+    bool ok[CH];
+    R xr[CH], xi[CH], o[CH], oi[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int e = threadIdx.x + 256 * (c0 + j);
+      const int64_t row = tm * BM + e / BN, col = tn * BN + e % BN;
+      ok[j] = row < g.M && col < g.N;
+      idx[j] = ok[j] ? row * g.N + col : 0;
+      cidx[j] = ok[j] ? g.c_base + g.mC[row] + g.nC[col] : 0;
+      xr[j] = part[idx[j]];
+      xi[j] = CPLX ? part[plane + idx[j]] : (R)0;
+    }
+    for (int s = 1; s < g.k_splits; ++s) {
+      R yr[CH], yi[CH];
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        yr[j] = part[((int64_t)s * P) * plane + idx[j]];
+        yi[j] = CPLX ? part[((int64_t)s * P + 1) * plane + idx[j]] : (R)0;
+      }
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        xr[j] += yr[j];
+        if (CPLX) xi[j] += yi[j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      o[j] = 0;
+      oi[j] = 0;
+      if (ok[j]) load_c_old<R, CPLX, MIXED>(g, cidx[j], o[j], oi[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < CH; ++j)
+      if (ok[j]) store_c_at<R, CPLX, MIXED>(g, cidx[j], xr[j], xi[j], o[j], oi[j]);
+  }
+  if (threadIdx.x == 0) atomicExch(&g.tile_counters[tile], 0);
+}
 
 template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED>
 __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmArgs g) {
   typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, MIXED> TG;
   __shared__ R smem[TG::SMEM];
+  __shared__ int last;
   const int64_t tiles_m = (g.M + BM - 1) / BM;
   const int64_t tiles_n = (g.N + BN - 1) / BN;
   const int64_t bid = xcd_swizzle(blockIdx.x, (int64_t)gridDim.x);
   const int64_t tm = bid % tiles_m, tn = bid / tiles_m;
   if (tn >= tiles_n) return;
-  TG::run(g, tm, tn, blockIdx.y, smem);
-}
-
-// Split-K reduction: partial sums added in split order 0, 1, ..., S-1 (a
-// fixed order, so results do not depend on scheduling), then the epilogue.
-// Columns across threads (the partials are row-major, so loads coalesce),
-// rows across blockIdx.y with a grid-stride loop.
-template <typename R, bool CPLX, bool MIXED>
-__global__ __launch_bounds__(256) void splitk_reduce_kernel(const GemmArgs g) {
-  constexpr int P = CPLX ? 2 : 1;
-  const int64_t plane = g.M * g.N;
-  const R *part = (const R *)g.partials;
-  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (col >= g.N) return;
-  for (int64_t row = blockIdx.y; row < g.M; row += gridDim.y) {
-    const int64_t e = row * g.N + col;
-    R xr = part[e], xi = CPLX ? part[plane + e] : (R)0;
-    for (int s = 1; s < g.k_splits; ++s) {
-      xr += part[((int64_t)s * P) * plane + e];
-      if (CPLX) xi += part[((int64_t)s * P + 1) * plane + e];
-    }
-    store_c<R, CPLX, MIXED>(g, row, col, xr, xi);
-  }
+  TG::template run<true>(g, tm, tn, blockIdx.y, smem);
+  if (g.k_splits > 1) splitk_fixup<R, CPLX, MIXED, BM, BN>(g, bid, tm, tn, &last);
 }
 
 template <typename R, bool CPLX, int BM, int BN, bool MIXED>
@@ -432,11 +627,6 @@ static hipError_t launch_tile_mixed(const GemmArgs &g, hipStream_t s) {
     hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, false, true, MIXED>), grid, block, 0, s, g);
   else
     hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, false, false, MIXED>), grid, block, 0, s, g);
-  hipError_t err = hipGetLastError();
-  if (err != hipSuccess || g.k_splits == 1) return err;
-  const int64_t bx = (g.N + 255) / 256;
-  const int64_t by = g.M < 65535 ? g.M : 65535;
-  hipLaunchKernelGGL((splitk_reduce_kernel<R, CPLX, MIXED>), dim3((unsigned)bx, (unsigned)by), dim3(256), 0, s, g);
   return hipGetLastError();
 }
 

@@ -69,9 +69,9 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
     g.beta_case = (c[2] == 0 && c[3] == 0) ? 0 : (c[2] == 1 && c[3] == 0) ? 1 : 2;
     if (c[0] == 0 && c[1] == 0) g.K = 0;   // alpha == 0: C = beta C, A and B never read
     switch (items[lo].cls) {
-      case TILE_128x64: TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::run(g, tm, tn, 0, smem); break;
-      case TILE_32x64: TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::run(g, tm, tn, 0, smem); break;
-      default: TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::run(g, tm, tn, 0, smem); break;
+      case TILE_128x64: TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::template run<false>(g, tm, tn, 0, smem); break;
+      case TILE_32x64: TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::template run<false>(g, tm, tn, 0, smem); break;
+      default: TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::template run<false>(g, tm, tn, 0, smem); break;
     }
   }
 }
@@ -177,6 +177,7 @@ static qs_status build_tables(qs_group_plan *g) {
     a.k_splits = 1;
     a.k_chunk = p->K;
     a.partials = nullptr;
+    a.tile_counters = nullptr;
     GroupItem it;
     std::memset(&it, 0, sizeof it);
     it.g = a;

@@ -47,14 +47,15 @@ struct GemmArgs {
   int beta_case;          // 0: beta == 0, 1: beta == 1, 2: general
   int k_splits;           // > 1: partial sums into `partials`, reduced in a fixed order
   int64_t k_chunk;        // K steps per split (multiple of the tile's BK)
-  void *partials;         // [k_splits][planes][N][M] of the real compute type
+  void *partials;         // [k_splits][planes][M][N] of the real compute type
   int tile;               // TileId
   int a_kfast, b_kfast;   // gather lanes along K (1) or along M / N (0)
   int mixed;              // some storage type differs from the compute type
+  int *tile_counters;     // k_splits > 1: per-tile arrivals, zero between launches
 };
 
-// Launch the tiled MFMA GEMM (and the fixed-order split-K reduction when
-// k_splits > 1).  Shapes and tables must already be validated.
+// Launch the tiled MFMA GEMM (with k_splits > 1, the last split of each tile
+// reduces the partials in a fixed order).  Shapes and tables must already be validated.
 hipError_t launch_gemm(const GemmArgs &g, hipStream_t stream);
 
 // C = beta * C over the M x N rectangle (alpha == 0 or K == 0).

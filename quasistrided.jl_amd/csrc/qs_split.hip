@@ -472,31 +472,42 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
     }
   }
   __syncthreads();
-  if (MFAST) {
-    const int64_t row = mb0 * 16 + lane;
-    if (row < g.M) {
-      const RowScale rs = a.sa[row];
-      for (int cc = 0; cc < 16; ++cc) {
-        const int64_t col = nb0 * 16 + cc;
-        if (col >= g.N) break;
-        const RowScale cs = a.sb[col];
-        const double v = ldexp(tile[wave][lane][cc] * rs.iq * cs.iq, rs.sh + cs.sh);
-        if (g.compute_type == QS_F64) store_c<double, false, true>(g, row, col, v, 0.0);
-        else store_c<float, false, true>(g, row, col, (float)v, 0.0f);
+  // Lane `lane` finishes 16 elements along C's unit-stride side: all 16
+  // old-C loads are issued before the first store.
+  const int64_t fixed = MFAST ? mb0 * 16 + lane : nb0 * 16 + lane;
+  if (fixed >= (MFAST ? g.M : g.N)) return;
+  const RowScale fs = MFAST ? a.sa[fixed] : a.sb[fixed];
+  const int64_t foff = MFAST ? g.mC[fixed] : g.nC[fixed];
+#pragma unroll
+  for (int h = 0; h < 16; h += 8) {
+    int64_t idx[8];
+    bool ok[8];
+    double v[8], o[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = h + j;
+      const int64_t other = (MFAST ? nb0 : mb0) * 16 + e;
+      ok[j] = other < (MFAST ? g.N : g.M);
+      const RowScale os = ok[j] ? (MFAST ? a.sb[other] : a.sa[other]) : RowScale{1.0, 0, 0};
+      idx[j] = ok[j] ? g.c_base + foff + (MFAST ? g.nC[other] : g.mC[other]) : 0;
+      v[j] = ldexp((MFAST ? tile[wave][lane][e] : tile[wave][e][lane]) * fs.iq * os.iq, fs.sh + os.sh);
+      o[j] = 0.0;
+      if (ok[j]) {
+        if (g.compute_type == QS_F64) {
+          double oi;
+          load_c_old<double, false, true>(g, idx[j], o[j], oi);
+        } else {
+          float of, ofi;
+          load_c_old<float, false, true>(g, idx[j], of, ofi);
+          o[j] = of;
+        }
       }
     }
-  } else {
-    const int64_t col = nb0 * 16 + lane;
-    if (col < g.N) {
-      const RowScale cs = a.sb[col];
-      for (int rr = 0; rr < 16; ++rr) {
-        const int64_t row = mb0 * 16 + rr;
-        if (row >= g.M) break;
-        const RowScale rs = a.sa[row];
-        const double v = ldexp(tile[wave][rr][lane] * rs.iq * cs.iq, rs.sh + cs.sh);
-        if (g.compute_type == QS_F64) store_c<double, false, true>(g, row, col, v, 0.0);
-        else store_c<float, false, true>(g, row, col, (float)v, 0.0f);
-      }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (!ok[j]) continue;
+      if (g.compute_type == QS_F64) store_c_at<double, false, true>(g, idx[j], v[j], 0.0, o[j], 0.0);
+      else store_c_at<float, false, true>(g, idx[j], (float)v[j], 0.0f, (float)o[j], 0.0f);
     }
   }
 }
