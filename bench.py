@@ -86,7 +86,7 @@ def config4(qs, torch, dev):
 
 def bench_split(qs, torch, dev, d=64, reps=5):
     """Rank-4 all-64 F64 with entries N(0,1) * 2^u (u integer in [-20, 20] per
-    row of A and per column of B): plain FP64 MFMA vs split precision (INT8
+    row of A and per column of B): autotuned plain FP64 MFMA vs split precision (INT8
     residue GEMMs), both timed per execute; the split error from the sampled Dot2
     report."""
     rng = np.random.default_rng(4)
@@ -107,6 +107,7 @@ def bench_split(qs, torch, dev, d=64, reps=5):
     mk = lambda Cst, prec: qs.plan_contract(SV(Cst, (d,) * 4, st), SV(A, (d,) * 4, st), (1, 5, 2, 6),
                                             SV(B, (d,) * 4, st), (5, 3, 6, 4), (1, 2, 3, 4), precision=prec)
     plain = mk(C1, "plain")
+    qs.autotune(plain, reps=3)   # the plain side at its best tile, as in the rank-4 bench
     split = mk(C2, "split")
     flops = 2 * d ** 6
     out = {}
@@ -124,6 +125,49 @@ def bench_split(qs, torch, dev, d=64, reps=5):
                 "split_vs_plain_rel_fro": diff,
                 "int8_pops_equiv": round(flops * rep["moduli"] / (out["split_ms"] * 1e-3)
                                          / 1e15, 3)})
+    return out
+
+
+def bench_bandwidth(qs, torch, dev, reps=20):
+    """The HBM-bound paths at sizes well past the 4 MB L2 of an XCD (and the
+    MALL for the large operand), F64, operands N(0,1): algorithmic bytes per
+    execute over the HIP-event kernel time on the launch stream, against
+    8 TB/s.
+      dot_kfast   C[m] := A[m,k] * b[k], A row-major (K contiguous), M=65536, K=2048
+      dot_mfast   the same with A column-major (M contiguous)
+      outer       C[m,n] := a[m] * b[n], M=N=8192, beta=0 (C written, never read)
+      skinny      C[m,n] := A[m,k] * B[k,n], M=16, K=4096, N=32768, B column-major
+                  (read in place, streamed once)"""
+    SV = qs.StridedView
+    g = torch.Generator(device=dev)
+    g.manual_seed(9)
+    rnd = lambda n: torch.randn(n, dtype=torch.float64, device=dev, generator=g)
+    out = {}
+
+    def run(name, plan, nbytes, flops, beta=0.0):
+        _, km, _ = time_plan(qs, torch, plan, beta, reps, 3)
+        out[name] = {"path": plan.path, "kernel_us": round(km * 1e6, 2), "gb_per_s": round(nbytes / km / 1e9, 1),
+                     "hbm_frac": round(nbytes / km / PEAK_HBM, 4), "bytes": nbytes,
+                     "gflops": round(flops / km / 1e9, 1)}
+
+    M, K = 65536, 2048
+    A, b, c = rnd(M * K), rnd(K), torch.zeros(M, dtype=torch.float64, device=dev)
+    for name, strides in (("dot_kfast", (K, 1)), ("dot_mfast", (1, M))):
+        plan = qs.plan_contract(SV(c, (M,), (1,)), SV(A, (M, K), strides), (1, 2), SV(b, (K,), (1,)), (2,), (1,))
+        run(name, plan, 8 * (M * K + K + M), 2 * M * K)
+    del A, b, c
+    M = N = 8192
+    a, b, C = rnd(M), rnd(N), torch.zeros(M * N, dtype=torch.float64, device=dev)
+    plan = qs.plan_contract(SV(C, (M, N), (1, M)), SV(a, (M,), (1,)), (1,), SV(b, (N,), (1,)), (2,), (1, 2))
+    run("outer", plan, 8 * (M + N + M * N), 2 * M * N)
+    del a, b, C
+    M, K, N = 16, 4096, 32768
+    A, B, C = rnd(M * K), rnd(K * N), torch.zeros(M * N, dtype=torch.float64, device=dev)
+    plan = qs.plan_contract(SV(C, (M, N), (1, M)), SV(A, (M, K), (1, M)), (1, 2), SV(B, (K, N), (1, K)), (2, 3),
+                            (1, 3))
+    qs.autotune(plan, reps=3)
+    run("skinny", plan, 8 * (M * K + K * N + M * N), 2 * M * K * N)
+    del A, B, C
     return out
 
 
@@ -365,7 +409,7 @@ def main():
     ap.add_argument("--tune-cache", default=None,
                     help="autotune cache file: loaded if it exists (no tuning runs), else written after tuning")
     ap.add_argument("--only", default=None,
-                    choices=["split", "grouped_f64", "grouped_c64", "served"],
+                    choices=["split", "grouped_f64", "grouped_c64", "served", "bandwidth"],
                     help="only run one of the extra workloads and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy 5 times (PMC byte-counter calibration)")
@@ -394,6 +438,8 @@ def main():
     if args.only is not None:
         if args.only == "split":
             out = bench_split(qs, torch, dev)
+        elif args.only == "bandwidth":
+            out = bench_bandwidth(qs, torch, dev)
         elif args.only.startswith("grouped"):
             out = bench_grouped(qs, torch, dev, args.only.endswith("c64"))
         else:
@@ -487,6 +533,7 @@ def main():
                            "mfma_frac": round(fl / km / PEAK_FP64, 4), "plan": repr(p)}
             del p, vw
         extra["split_rank4_all64_f64"] = bench_split(qs, torch, dev)
+        extra["bandwidth_f64"] = bench_bandwidth(qs, torch, dev)
         extra["grouped_512_mps_f64"] = bench_grouped(qs, torch, dev, False)
         extra["grouped_512_mps_c64"] = bench_grouped(qs, torch, dev, True)
         pools = served_pools(qs, torch, dev)

@@ -62,56 +62,128 @@ struct DotArgs {
 };
 
 template <typename R, bool CPLX, bool MIXED>
-__device__ __forceinline__ void dot_step(const DotArgs &d, int64_t xrow, int64_t k, R &accr, R &acci) {
-  R xr, xi, vr, vi;
-  load_elem<R, CPLX, MIXED>(d.X, d.x_type, xrow + d.kX[k], xr, xi);
-  load_elem<R, CPLX, MIXED>(d.V, d.v_type, d.vbase + d.kv[k], vr, vi);
-  if (!CPLX) {
-    accr = qs_fma(xr, vr, accr);
-  } else {
-    if (d.conj_x) xi = -xi;
-    if (d.conj_v) vi = -vi;
-    const R nr = qs_fma(-xi, vi, qs_fma(xr, vr, accr));
-    const R ni = qs_fma(xi, vr, qs_fma(xr, vi, acci));
-    accr = nr;
-    acci = ni;
-  }
-}
-
-template <typename R, bool CPLX, bool MIXED>
 __device__ __forceinline__ void dot_store(const GemmArgs &g, const DotArgs &d, int64_t q, R xr, R xi) {
   if (d.rows_are_m) store_c<R, CPLX, MIXED>(g, q, 0, xr, xi);
   else store_c<R, CPLX, MIXED>(g, 0, q, xr, xi);
 }
 
-// One wave per output, lanes along K (the matrix is K-contiguous): each lane
-// sums its own K residue class in order, then a fixed butterfly reduction.
+// Lanes along K (the matrix is K-contiguous): one wave per 4 outputs.  Each
+// lane sums its own K residue class (k = lane mod 64) of each of the 4 rows
+// in order, with 4 k-steps of loads issued together; the K offsets and the
+// vector elements of a step are loaded once for the 4 rows.  Then a fixed
+// butterfly reduction per row.
 template <typename R, bool CPLX, bool MIXED>
 __global__ __launch_bounds__(256) void dot_klanes_kernel(const GemmArgs g, const DotArgs d) {
+  constexpr int RW = 4, U = 4;
   const int lane = threadIdx.x & 63;
-  const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (q >= d.Q) return;
-  const int64_t xrow = d.xbase + d.fX[q];
-  R accr = 0, acci = 0;
-  for (int64_t k = lane; k < d.K; k += 64) dot_step<R, CPLX, MIXED>(d, xrow, k, accr, acci);
+  const int64_t q0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RW;
+  if (q0 >= d.Q) return;
+  int64_t xrow[RW];
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    accr += __shfl_xor(accr, off, 64);
-    if (CPLX) acci += __shfl_xor(acci, off, 64);
+  for (int r = 0; r < RW; ++r) xrow[r] = d.xbase + d.fX[q0 + r < d.Q ? q0 + r : d.Q - 1];
+  R accr[RW], acci[RW];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) accr[r] = acci[r] = 0;
+  for (int64_t k0 = 0; k0 < d.K; k0 += 64 * U) {
+    R xr[U][RW], xi[U][RW], vr[U], vi[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = k0 + 64 * u + lane;
+      ok[u] = k < d.K;
+      const int64_t kc = ok[u] ? k : 0;
+      const int64_t kx = d.kX[kc];
+      load_elem<R, CPLX, MIXED>(d.V, d.v_type, d.vbase + d.kv[kc], vr[u], vi[u]);
+#pragma unroll
+      for (int r = 0; r < RW; ++r) load_elem<R, CPLX, MIXED>(d.X, d.x_type, xrow[r] + kx, xr[u][r], xi[u][r]);
+    }
+    __builtin_amdgcn_sched_barrier(0);   // every load of the step is in flight
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const R v_r = vr[u], v_i = d.conj_v ? -vi[u] : vi[u];
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        if (!CPLX) {
+          const R nr = qs_fma(xr[u][r], v_r, accr[r]);
+          accr[r] = ok[u] ? nr : accr[r];
+        } else {
+          const R x_i = d.conj_x ? -xi[u][r] : xi[u][r];
+          const R nr = qs_fma(-x_i, v_i, qs_fma(xr[u][r], v_r, accr[r]));
+          const R ni = qs_fma(x_i, v_r, qs_fma(xr[u][r], v_i, acci[r]));
+          accr[r] = ok[u] ? nr : accr[r];
+          acci[r] = ok[u] ? ni : acci[r];
+        }
+      }
+    }
   }
-  if (lane == 0) dot_store<R, CPLX, MIXED>(g, d, q, accr, acci);
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      accr[r] += __shfl_xor(accr[r], off, 64);
+      if (CPLX) acci[r] += __shfl_xor(acci[r], off, 64);
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+      if (q0 + r < d.Q) dot_store<R, CPLX, MIXED>(g, d, q0 + r, accr[r], acci[r]);
+  }
 }
 
-// One thread per output, lanes along the free extent (the matrix is
-// free-contiguous): every thread walks K in order, as the scalar kernel does.
+// Lanes along the free extent (the matrix is free-contiguous): 64 outputs per
+// workgroup, each wave one contiguous quarter of K.  A thread walks its
+// quarter in order with 8 k-steps of loads issued together (the K offsets
+// and vector elements are wave-uniform); the four quarters are then added in
+// order 0..3.
 template <typename R, bool CPLX, bool MIXED>
 __global__ __launch_bounds__(256) void dot_qlanes_kernel(const GemmArgs g, const DotArgs d) {
-  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (q >= d.Q) return;
-  const int64_t xrow = d.xbase + d.fX[q];
+  constexpr int S = 4, U = 8;
+  __shared__ R part[S][64][CPLX ? 2 : 1];
+  const int lane = threadIdx.x & 63;
+  const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t q = (int64_t)blockIdx.x * 64 + lane;
+  const bool qok = q < d.Q;
+  const int64_t xrow = d.xbase + d.fX[qok ? q : d.Q - 1];
+  const int64_t chunk = (d.K + S - 1) / S;
+  const int64_t kb = s * chunk, ke = kb + chunk < d.K ? kb + chunk : d.K;
   R accr = 0, acci = 0;
-  for (int64_t k = 0; k < d.K; ++k) dot_step<R, CPLX, MIXED>(d, xrow, k, accr, acci);
-  dot_store<R, CPLX, MIXED>(g, d, q, accr, acci);
+  for (int64_t k0 = kb; k0 < ke; k0 += U) {
+    R xr[U], xi[U], vr[U], vi[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = k0 + u < ke ? k0 + u : ke - 1;
+      load_elem<R, CPLX, MIXED>(d.V, d.v_type, d.vbase + ((qs_const_i64)d.kv)[k], vr[u], vi[u]);
+      load_elem<R, CPLX, MIXED>(d.X, d.x_type, xrow + ((qs_const_i64)d.kX)[k], xr[u], xi[u]);
+    }
+    __builtin_amdgcn_sched_barrier(0);   // every load of the step is in flight
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool in = k0 + u < ke;
+      if (!CPLX) {
+        const R nr = qs_fma(xr[u], vr[u], accr);
+        accr = in ? nr : accr;
+      } else {
+        const R x_i = d.conj_x ? -xi[u] : xi[u], v_i = d.conj_v ? -vi[u] : vi[u];
+        const R nr = qs_fma(-x_i, v_i, qs_fma(xr[u], vr[u], accr));
+        const R ni = qs_fma(x_i, vr[u], qs_fma(xr[u], v_i, acci));
+        accr = in ? nr : accr;
+        acci = in ? ni : acci;
+      }
+    }
+  }
+  part[s][lane][0] = accr;
+  if (CPLX) part[s][lane][CPLX ? 1 : 0] = acci;
+  __syncthreads();
+  if (s == 0 && qok) {
+    R tr = part[0][lane][0], ti = CPLX ? part[0][lane][CPLX ? 1 : 0] : (R)0;
+#pragma unroll
+    for (int w = 1; w < S; ++w) {
+      tr += part[w][lane][0];
+      if (CPLX) ti += part[w][lane][CPLX ? 1 : 0];
+    }
+    dot_store<R, CPLX, MIXED>(g, d, q, tr, ti);
+  }
 }
 
 static DotArgs make_dot(const GemmArgs &g) {
@@ -139,48 +211,86 @@ static hipError_t launch_dot_typed(const GemmArgs &g, hipStream_t s) {
   // the caller folded it into `vbase` (see qs_api.cpp).
   const bool klanes = g.N == 1 ? g.a_kfast : g.b_kfast;
   if (klanes) {
-    const int64_t blocks = (d.Q + 3) / 4;
+    const int64_t blocks = (d.Q + 15) / 16;
     hipLaunchKernelGGL((dot_klanes_kernel<R, CPLX, MIXED>), dim3((unsigned)blocks), dim3(256), 0, s, g, d);
   } else {
-    const int64_t blocks = (d.Q + 255) / 256;
+    const int64_t blocks = (d.Q + 63) / 64;
     hipLaunchKernelGGL((dot_qlanes_kernel<R, CPLX, MIXED>), dim3((unsigned)blocks), dim3(256), 0, s, g, d);
   }
   return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ outer
-// K == 1: one thread per C element, threads laid along C's unit-stride side
-// (`a_kfast` carries "N fastest" for this path).
+// K == 1, in strips of 256 x 16 elements of C: threads laid along C's
+// unit-stride side (`a_kfast` carries "N fastest" for this path), each
+// thread owning one row (column) and walking 16 columns (rows) whose offsets
+// and operand elements are workgroup-uniform.  Per thread, the 16 old-C
+// loads are issued before the first store.
 template <typename R, bool CPLX, bool MIXED, bool NFAST>
 __global__ __launch_bounds__(256) void outer_kernel(const GemmArgs g) {
-  const int64_t n = g.M * g.N;
-  for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
-    const int64_t i = NFAST ? e / g.N : e % g.M;
-    const int64_t j = NFAST ? e % g.N : e / g.M;
-    R ar, ai, br, bi;
-    load_elem<R, CPLX, MIXED>(g.A, g.a_type, g.a_base + g.mA[i] + g.kA[0], ar, ai);
-    load_elem<R, CPLX, MIXED>(g.B, g.b_type, g.b_base + g.nB[j] + g.kB[0], br, bi);
-    R xr, xi = 0;
-    if (!CPLX) {
-      xr = qs_fma(ar, br, (R)0);
+  constexpr int W = 16;
+  const int64_t fast = NFAST ? g.N : g.M, slow = NFAST ? g.M : g.N;
+  const int64_t f = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (f >= fast) return;
+  // The fast side's operand element and C offset, once per thread.
+  R fr, fi;
+  int64_t foff;
+  if (NFAST) {
+    load_elem<R, CPLX, MIXED>(g.B, g.b_type, g.b_base + g.nB[f] + g.kB[0], fr, fi);
+    if (CPLX && g.conj_b) fi = -fi;
+    foff = g.nC[f];
+  } else {
+    load_elem<R, CPLX, MIXED>(g.A, g.a_type, g.a_base + g.mA[f] + g.kA[0], fr, fi);
+    if (CPLX && g.conj_a) fi = -fi;
+    foff = g.mC[f];
+  }
+  for (int64_t s0 = (int64_t)blockIdx.y * W; s0 < slow; s0 += (int64_t)gridDim.y * W) {
+  int64_t idx[W];
+  bool ok[W];
+  R xr[W], xi[W], o[W], oi[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const int64_t sl = s0 + w;
+    ok[w] = sl < slow;
+    const int64_t sc = ok[w] ? sl : slow - 1;
+    R sr, si;
+    // sc is workgroup-uniform: its table entries come through the scalar cache.
+    if (NFAST) {
+      load_elem<R, CPLX, MIXED>(g.A, g.a_type, g.a_base + ((qs_const_i64)g.mA)[sc] + ((qs_const_i64)g.kA)[0], sr, si);
+      if (CPLX && g.conj_a) si = -si;
+      idx[w] = g.c_base + ((qs_const_i64)g.mC)[sc] + foff;
     } else {
-      if (g.conj_a) ai = -ai;
-      if (g.conj_b) bi = -bi;
-      xr = qs_fma(-ai, bi, qs_fma(ar, br, (R)0));
-      xi = qs_fma(ai, br, qs_fma(ar, bi, (R)0));
+      load_elem<R, CPLX, MIXED>(g.B, g.b_type, g.b_base + ((qs_const_i64)g.nB)[sc] + ((qs_const_i64)g.kB)[0], sr, si);
+      if (CPLX && g.conj_b) si = -si;
+      idx[w] = g.c_base + foff + ((qs_const_i64)g.nC)[sc];
     }
-    store_c<R, CPLX, MIXED>(g, i, j, xr, xi);
+    // a_i * b_j in the order the scalar kernel forms it (A's element first).
+    const R ar = NFAST ? sr : fr, ai = NFAST ? si : fi, br = NFAST ? fr : sr, bi = NFAST ? fi : si;
+    if (!CPLX) {
+      xr[w] = qs_fma(ar, br, (R)0);
+      xi[w] = 0;
+    } else {
+      xr[w] = qs_fma(-ai, bi, qs_fma(ar, br, (R)0));
+      xi[w] = qs_fma(ai, br, qs_fma(ar, bi, (R)0));
+    }
+    load_c_old<R, CPLX, MIXED>(g, idx[w], o[w], oi[w]);   // idx is valid: clamped
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    if (ok[w]) store_c_at<R, CPLX, MIXED>(g, idx[w], xr[w], xi[w], o[w], oi[w]);
   }
 }
 
 template <typename R, bool CPLX, bool MIXED>
 static hipError_t launch_outer_typed(const GemmArgs &g, hipStream_t s) {
-  int64_t blocks = (g.M * g.N + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
+  const int64_t fast = g.a_kfast ? g.N : g.M, slow = g.a_kfast ? g.M : g.N;
+  const int64_t gy = (slow + 15) / 16;
+  const dim3 grid((unsigned)((fast + 255) / 256), (unsigned)(gy < 65535 ? gy : 65535));
   if (g.a_kfast)
-    hipLaunchKernelGGL((outer_kernel<R, CPLX, MIXED, true>), dim3((unsigned)blocks), dim3(256), 0, s, g);
+    hipLaunchKernelGGL((outer_kernel<R, CPLX, MIXED, true>), grid, dim3(256), 0, s, g);
   else
-    hipLaunchKernelGGL((outer_kernel<R, CPLX, MIXED, false>), dim3((unsigned)blocks), dim3(256), 0, s, g);
+    hipLaunchKernelGGL((outer_kernel<R, CPLX, MIXED, false>), grid, dim3(256), 0, s, g);
   return hipGetLastError();
 }
 
