@@ -137,7 +137,9 @@ def bench_bandwidth(qs, torch, dev, reps=20):
       dot_mfast   the same with A column-major (M contiguous)
       outer       C[m,n] := a[m] * b[n], M=N=8192, beta=0 (C written, never read)
       skinny      C[m,n] := A[m,k] * B[k,n], M=16, K=4096, N=32768, B column-major
-                  (read in place, streamed once)"""
+                  (read in place, streamed once)
+      transpose_add  tensoradd! C = permutedims(A, (2,1)), 8192 x 8192, beta=0
+                  (timed per call, plan creation included)"""
     SV = qs.StridedView
     g = torch.Generator(device=dev)
     g.manual_seed(9)
@@ -168,6 +170,25 @@ def bench_bandwidth(qs, torch, dev, reps=20):
     qs.autotune(plan, reps=3)
     run("skinny", plan, 8 * (M * K + K * N + M * N), 2 * M * K * N)
     del A, B, C
+    # tensoradd! transposing 8192 x 8192 (beta = 0: A read once, C written once).
+    n = 8192
+    A, Cst = rnd(n * n), torch.empty(n * n, dtype=torch.float64, device=dev)
+    Av, Cv = SV(A, (n, n), (1, n)), SV(Cst, (n, n), (1, n))
+    for _ in range(3):
+        qs.tensoradd(Cv, Av, ((2, 1), ()), False, 1.0, 0.0)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record()
+        qs.tensoradd(Cv, Av, ((2, 1), ()), False, 1.0, 0.0)
+        e1.record()
+    torch.cuda.synchronize()
+    t = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev])) * 1e-3
+    nbytes = 2 * 8 * n * n
+    out["transpose_add"] = {"path": "tensoradd", "call_us": round(t * 1e6, 2), "gb_per_s": round(nbytes / t / 1e9, 1),
+                            "hbm_frac": round(nbytes / t / PEAK_HBM, 4), "bytes": nbytes,
+                            "note": "per call: plan, table upload, launch and release included"}
+    del A, Cst
     return out
 
 

@@ -179,6 +179,33 @@ qs_status qs_plan_rebind(qs_plan *plan, const void *a, int64_t a_base, int64_t a
  * (src/execution/workspace.jl:110-123).  NULL is a no-op. */
 qs_status qs_plan_destroy(qs_plan *plan);
 
+/* ---- permuted add and partial trace ----------------------------------------
+ * Device counterparts of TensorOperations' `tensoradd!` / `tensortrace!`,
+ * which the reference forwards to `StridedNative`
+ * (src/integrations/tensoroperations.jl:90-107):
+ *
+ *     C[m, n] = alpha * sum_k op(A)[m, n, k] + beta * C[m, n]
+ *
+ * over the groups M and N (two maps each: A, then C) and K, the traced pairs
+ * (map 0: the sum of the pair's two strides in A; map 1 unused).  The host
+ * puts C's unit-stride dim first in M and A's first in N; a transposing
+ * permutation is then coalesced on both sides.  op(A) = conj(A) when
+ * `conj_a`.  Device memory only; alpha == 0 or K == 0 never reads A, beta ==
+ * 0 never reads C; exact span checks before any write, as for contractions. */
+typedef struct qs_add_desc {
+  int32_t a_type, c_type, compute_type, conj_a;
+  qs_group m, n, k;
+  const void *a; int64_t a_base; int64_t a_len;
+  void *c;       int64_t c_base; int64_t c_len;
+} qs_add_desc;
+
+typedef struct qs_add_plan qs_add_plan;
+
+qs_status qs_add_plan_create(const qs_add_desc *desc, qs_add_plan **out);
+/* alpha and beta hold one value of the compute type (complex: 2 reals). */
+qs_status qs_add_execute(qs_add_plan *plan, const void *alpha, const void *beta, void *stream);
+qs_status qs_add_plan_destroy(qs_add_plan *plan);
+
 /* ---- split precision ------------------------------------------------------
  * No counterpart in the reference.  A plan created with precision
  * QS_SPLIT_INT8 (real compute types) runs Ozaki scheme II on the INT8 MFMA

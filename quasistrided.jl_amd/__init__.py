@@ -14,6 +14,7 @@ from .plan import (ContractPlan, MFMAKernel, PathModes, autotune, autotune_clear
                    compute_type, contract, contract_, error_report, execute, execute_, isconj, op_conjugates, plan_contract, planned,
                    release, set_error_report)
 from .backend import QuasiStridedBackend, contraction_labels, tensorcontract
+from .add import tensoradd, tensortrace
 from .target import target_profile
 from .group import GroupPlan, Graph
 from .server import Server
@@ -25,5 +26,6 @@ __all__ = [
     "map_ramp_step", "order_contract_labels", "order_free_labels", "prefer_swap", "ContractPlan", "MFMAKernel",
     "PathModes", "autotune", "autotune_clear", "autotune_load", "autotune_save", "compute_type", "contract", "contract_", "error_report", "set_error_report", "execute", "execute_", "isconj", "op_conjugates",
     "plan_contract", "planned", "release", "QuasiStridedBackend", "contraction_labels", "tensorcontract",
+    "tensoradd", "tensortrace",
     "target_profile", "GroupPlan", "Graph", "Server",
 ]

@@ -90,6 +90,15 @@ class QSTargetProfile(C.Structure):
     ]
 
 
+class QSAddDesc(C.Structure):
+    _fields_ = [
+        ("a_type", C.c_int32), ("c_type", C.c_int32), ("compute_type", C.c_int32), ("conj_a", C.c_int32),
+        ("m", QSGroup), ("n", QSGroup), ("k", QSGroup),
+        ("a", C.c_void_p), ("a_base", C.c_int64), ("a_len", C.c_int64),
+        ("c", C.c_void_p), ("c_base", C.c_int64), ("c_len", C.c_int64),
+    ]
+
+
 # Every symbol include/qs_amd.h declares: name -> (restype, argtypes).
 SIGNATURES = {
     "qs_last_error": (C.c_char_p, []),
@@ -101,6 +110,9 @@ SIGNATURES = {
     "qs_plan_rebind": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
                                  C.c_void_p, C.c_int64, C.c_int64]),
     "qs_plan_destroy": (C.c_int, [C.c_void_p]),
+    "qs_add_plan_create": (C.c_int, [C.POINTER(QSAddDesc), C.POINTER(C.c_void_p)]),
+    "qs_add_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "qs_add_plan_destroy": (C.c_int, [C.c_void_p]),
     "qs_plan_set_error_report": (C.c_int, [C.c_void_p, C.c_int32]),
     "qs_plan_get_error_report": (C.c_int, [C.c_void_p, C.POINTER(QSErrorReport)]),
     "qs_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(QSTuneResult)]),

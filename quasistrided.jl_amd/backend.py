@@ -7,7 +7,8 @@ one-based axis numbers as in Julia, turns them into labels exactly as
 """
 from __future__ import annotations
 
-from .errors import ArgumentError, DimensionMismatch, UnsupportedError
+from . import add as _add
+from .errors import ArgumentError, DimensionMismatch
 from .plan import compute_type, planned, execute
 from .strided import StridedView
 
@@ -82,13 +83,17 @@ class QuasiStridedBackend:
             plan.release()
         return C
 
-    def tensoradd(self, *args, **kwargs):
-        raise UnsupportedError("tensoradd!: the reference forwards it to TensorOperations.StridedNative, which "
-                               "this engine does not ship")
+    def tensoradd(self, C, A, pA, conjA, alpha, beta, allocator=None):
+        """TO.tensoradd! (tensoroperations.jl:91-98, forwarded to StridedNative
+        there): C = beta*C + alpha*permutedims(op(A), linearize(pA)) on the
+        device; returns C."""
+        return _add.tensoradd(C, A, pA, conjA, alpha, beta)
 
-    def tensortrace(self, *args, **kwargs):
-        raise UnsupportedError("tensortrace!: the reference forwards it to TensorOperations.StridedNative, which "
-                               "this engine does not ship")
+    def tensortrace(self, C, A, p, q, conjA, alpha, beta, allocator=None):
+        """TO.tensortrace! (tensoroperations.jl:100-107, forwarded to
+        StridedNative there): C = beta*C + alpha*(partial trace of op(A) over
+        the pairs q, permuted by linearize(p)) on the device; returns C."""
+        return _add.tensortrace(C, A, p, q, conjA, alpha, beta)
 
 
 def tensorcontract(C, A, pA, conjA, B, pB, conjB, pAB, alpha, beta, backend):

@@ -107,7 +107,7 @@ void map_range(const qs_group &g, int p, int64_t &lo, int64_t &hi) {
 
 // Move dim `d` of group `g` to the front (the enumeration order changes; the
 // set of (map0, map1) offset pairs does not).
-static void move_to_front(qs_group &g, int d) {
+void move_to_front(qs_group &g, int d) {
   if (d <= 0) return;
   const int64_t L = g.lengths[d], s0 = g.strides[0][d], s1 = g.strides[1][d];
   for (int e = d; e > 0; --e) {
@@ -121,7 +121,7 @@ static void move_to_front(qs_group &g, int d) {
 }
 
 // A non-singleton dim of map p with |stride| == 1, or -1.
-static int unit_dim(const qs_group &g, int p) {
+int unit_dim(const qs_group &g, int p) {
   for (int d = 0; d < g.ndim; ++d)
     if (g.lengths[d] > 1 && (g.strides[p][d] == 1 || g.strides[p][d] == -1)) return d;
   return -1;

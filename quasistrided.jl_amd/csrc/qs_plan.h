@@ -54,6 +54,9 @@ int64_t group_length(const qs_group &g);
 qs_status validate_group(const qs_group &g, const char *name);
 void fill_offsets(const qs_group &g, int64_t first, int64_t count, int64_t *o0, int64_t *o1);
 void map_range(const qs_group &g, int p, int64_t &lo, int64_t &hi);
+// Move dim d of a group to the front; a non-singleton dim with |stride| 1 in map p, or -1.
+void move_to_front(qs_group &g, int d);
+int unit_dim(const qs_group &g, int p);
 qs_status run_device(qs_plan *p, const void *A, int64_t a_base, const void *B, int64_t b_base, void *C, int64_t c_base,
                      const double *al, const double *be, hipStream_t s);
 qs_status run_staged(qs_plan *p, const double *al, const double *be, bool reads_ab, hipStream_t s);
