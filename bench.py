@@ -84,11 +84,11 @@ def config4(qs, torch, dev):
     return plan, 2 * 4096 ** 3, 3 * 8 * 4096 ** 2, (A, B, C), 0.0
 
 
-def bench_split(qs, torch, dev, d=64, reps=5):
+def bench_split(qs, torch, dev, d=64, reps=10):
     """Rank-4 all-64 F64 with entries N(0,1) * 2^u (u integer in [-20, 20] per
     row of A and per column of B): autotuned plain FP64 MFMA vs split precision (INT8
-    residue GEMMs), both timed per execute; the split error from the sampled Dot2
-    report."""
+    residue GEMMs), both timed per execute, sustained and interleaved; the split
+    error from the sampled Dot2 report."""
     rng = np.random.default_rng(4)
     SV = qs.StridedView
     ua = rng.integers(-20, 21, size=(d, d))
@@ -111,16 +111,25 @@ def bench_split(qs, torch, dev, d=64, reps=5):
     split = mk(C2, "split")
     flops = 2 * d ** 6
     out = {}
-    for name, p in (("plain", plain), ("split", split)):
-        _, km, _ = time_plan(qs, torch, p, 0.0, reps, 2)
+    # Sustained: each path back to back after about 50 ms of its own warm-up
+    # (long INT8 and FP64 MFMA runs move the clocks by 10-20 %, and the INT8
+    # GEMMs draw the most power).  Interleaved: executes alternate plain /
+    # split, both under the same mixed load.
+    for name, p, w in (("plain", plain, 20), ("split", split, 40)):
+        _, km, kmed = time_plan(qs, torch, p, 0.0, reps, w)
         out[name + "_ms"] = round(km * 1e3, 4)
+        out[name + "_ms_median"] = round(kmed * 1e3, 4)
         out[name + "_tflops_fp64_equiv"] = round(flops / km / 1e12, 2)
+    times = time_plans_alternating(qs, torch, (plain, split), 0.0, reps, 2)
+    for name, (km, _) in zip(("plain", "split"), times):
+        out[name + "_ms_interleaved"] = round(km * 1e3, 4)
     qs.set_error_report(split, True)
     qs.execute(split, 1.0, 0.0)
     torch.cuda.synchronize()
     rep = qs.error_report(split)
     diff = (C2 - C1).norm().item() / C1.norm().item()
     out.update({"moduli": rep["moduli"], "split_vs_plain_speedup": round(out["plain_ms"] / out["split_ms"], 3),
+                "split_vs_plain_speedup_interleaved": round(out["plain_ms_interleaved"] / out["split_ms_interleaved"], 3),
                 "split_rel_fro_err_sampled_dot2": rep["rel_fro"], "split_max_rel_err_sampled": rep["max_rel"],
                 "split_vs_plain_rel_fro": diff,
                 "int8_pops_equiv": round(flops * rep["moduli"] / (out["split_ms"] * 1e-3)
@@ -394,6 +403,29 @@ def time_plan(qs, torch, plan, beta, steps, warmup):
     wall = time.perf_counter() - t0
     per = [s.elapsed_time(e) * 1e-3 for s, e in zip(starts, ends)]
     return wall, float(np.mean(per)), float(np.median(per))
+
+
+def time_plans_alternating(qs, torch, plans, beta, reps, warmup):
+    """Warm up every plan, then time `reps` rounds of one execute of each plan
+    in turn with HIP events on the launch stream; (mean, median) seconds per plan."""
+    stream = torch.cuda.current_stream()
+    for p in plans:
+        for _ in range(warmup):
+            qs.execute(p, 1.0, beta)
+    torch.cuda.synchronize()
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+          for _ in plans]
+    for i in range(reps):
+        for j, p in enumerate(plans):
+            ev[j][i][0].record(stream)
+            qs.execute(p, 1.0, beta)
+            ev[j][i][1].record(stream)
+    torch.cuda.synchronize()
+    out = []
+    for j in range(len(plans)):
+        per = [a.elapsed_time(b) * 1e-3 for a, b in ev[j]]
+        out.append((float(np.mean(per)), float(np.median(per))))
+    return out
 
 
 def cpu_baseline(plan_views, seconds=10.0):

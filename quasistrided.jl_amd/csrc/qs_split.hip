@@ -58,6 +58,7 @@ struct Ssq {
 };
 
 struct RowScale {
+  double q;    // scale mantissa: A'_ik = rint(A_ik 2^-sh q)  (0 for a zero or non-finite row)
   double iq;   // 1 / q_i (NaN for a row holding Inf / NaN)
   int sh;      // power of two that undoes the scale
   int pad;
@@ -107,79 +108,129 @@ __device__ __forceinline__ Ssq ssq_merge(Ssq a, Ssq b) {
   return r;
 }
 
-// Pass 1: partial sums of squares of 64 rows over one K chunk of 256, in a
-// fixed order (the scales, hence the results, are reproducible bit for bit).
-// Loads are unconditional and issued 16 at a time: a lane outside the
-// operand reads its first element and discards it.
-__global__ __launch_bounds__(256) void crt_norm_kernel(CrtOperand o) {
-  __shared__ Ssq red[4][64];
+__device__ __forceinline__ int ssq_expo(double x) {
+  const double ax = fabs(x);
+  return (ax > 0.0 && ax <= DBL_MAX) ? ilogb(ax) : -2000;
+}
+
+// Pass 1: partial sums of squares of 64 rows over one K chunk of 256.  Every
+// load of a thread is issued before any arithmetic; each partial is scaled
+// by the largest exponent it holds (one ldexp per element, no serial
+// rescaling), and lanes and waves are combined by fixed butterflies, so the
+// scales, hence the results, are reproducible bit for bit.  Lanes outside
+// the operand read its first element and discard it.
+__global__ __launch_bounds__(512) void crt_norm_kernel(CrtOperand o) {
+  __shared__ Ssq red[8][64];
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int64_t r0 = (int64_t)blockIdx.x * 64;
   const int64_t kc0 = (int64_t)blockIdx.y * kNormChunk;
   const int64_t safe = o.base + o.rowtab[0] + o.ktab[0];
+  double im;
   if (o.kfast) {
-    // Lanes along K: row wave + 4 it for every lane of the wave.
-    Ssq acc[16];
+    // Lanes along K (k = kc0 + 64 j + lane), rows wave + 8 it of the group.
+    int64_t ko[4];
+    bool kin[4];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = Ssq{0.0, -1100, 0};
-#pragma unroll 1
-    for (int j = 0; j < kNormChunk / 64; ++j) {
+    for (int j = 0; j < 4; ++j) {
       const int64_t k = kc0 + 64 * j + lane;
-      const bool kin = k < o.K;
-      const int64_t ko = o.ktab[kin ? k : 0];
-      double x[16];
+      kin[j] = k < o.K;
+      ko[j] = o.ktab[kin[j] ? k : 0];
+    }
+    double x[8][4];
 #pragma unroll
-      for (int it = 0; it < 16; ++it) {
-        const int64_t r = r0 + wave + 4 * it;
-        const bool ok = kin && r < o.R;
-        double im;
-        load_elem<double, false, true>(o.X, o.type, ok ? o.base + o.rowtab[r] + ko : safe, x[it], im);
-        x[it] = ok ? x[it] : 0.0;
+    for (int it = 0; it < 8; ++it) {
+      const int64_t r = r0 + wave + 8 * it;
+      const bool rin = r < o.R;
+      const int64_t rb = o.base + o.rowtab[rin ? r : 0];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool ok = rin && kin[j];
+        load_elem<double, false, true>(o.X, o.type, ok ? rb + ko[j] : safe, x[it][j], im);
+        x[it][j] = ok ? x[it][j] : 0.0;
       }
-#pragma unroll
-      for (int it = 0; it < 16; ++it) ssq_add(acc[it], x[it]);
     }
 #pragma unroll
-    for (int it = 0; it < 16; ++it) {
-      Ssq a = acc[it];
-      for (int off = 32; off >= 1; off >>= 1) {   // fixed butterfly; lane 0's result is used
-        Ssq b;
-        b.s = __shfl_xor(a.s, off, 64);
-        b.e = __shfl_xor(a.e, off, 64);
-        a = ssq_merge(a, b);
+    for (int it = 0; it < 8; ++it) {
+      int e = max(max(ssq_expo(x[it][0]), ssq_expo(x[it][1])), max(ssq_expo(x[it][2]), ssq_expo(x[it][3])));
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) e = max(e, __shfl_xor(e, off, 64));
+      double sq = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double y = ldexp(fabs(x[it][j]), -e);
+        sq = __builtin_fma(y, y, sq);
       }
-      const int64_t r = r0 + wave + 4 * it;
-      if (lane == 0 && r < o.R) o.part[(int64_t)blockIdx.y * o.R + r] = a;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) sq += __shfl_xor(sq, off, 64);   // same sum in every lane
+      const int64_t r = r0 + wave + 8 * it;
+      if (lane == 0 && r < o.R) o.part[(int64_t)blockIdx.y * o.R + r] = Ssq{sq, e, 0};
     }
   } else {
-    // Lanes along the rows: each thread owns row `lane`, K steps wave + 4 i.
-    Ssq acc = Ssq{0.0, -1100, 0};
+    // Lanes along the rows (row r0 + lane), K steps wave + 8 i.
     const int64_t r = r0 + lane;
     const bool rin = r < o.R;
     const int64_t rbase = o.base + o.rowtab[rin ? r : 0];
-#pragma unroll 1
-    for (int j = 0; j < kNormChunk / 64; ++j) {
-      double x[16];
+    double x[32];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int64_t k = kc0 + wave + 4 * (16 * j + i);
-        const bool ok = rin && k < o.K;
-        double im;
-        load_elem<double, false, true>(o.X, o.type, ok ? rbase + o.ktab[k] : safe, x[i], im);
-        x[i] = ok ? x[i] : 0.0;
-      }
-#pragma unroll
-      for (int i = 0; i < 16; ++i) ssq_add(acc, x[i]);
+    for (int i = 0; i < 32; ++i) {
+      const int64_t k = kc0 + wave + 8 * i;
+      const bool ok = rin && k < o.K;
+      load_elem<double, false, true>(o.X, o.type, ok ? rbase + o.ktab[ok ? k : 0] : safe, x[i], im);
+      x[i] = ok ? x[i] : 0.0;
     }
-    red[wave][lane] = acc;
+    int ex[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) ex[i] = ssq_expo(x[i]);
+#pragma unroll
+    for (int w = 16; w >= 1; w >>= 1)
+#pragma unroll
+      for (int i = 0; i < w; ++i) ex[i] = max(ex[i], ex[i + w]);
+    const int e = ex[0];
+    double sq[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+      const double y = ldexp(fabs(x[i]), -e);
+      sq[i & 3] = __builtin_fma(y, y, sq[i & 3]);
+    }
+    red[wave][lane] = Ssq{(sq[0] + sq[1]) + (sq[2] + sq[3]), e, 0};
     __syncthreads();
     if (t < 64) {
       Ssq a = red[0][t];
-      for (int w = 1; w < 4; ++w) a = ssq_merge(a, red[w][t]);
+#pragma unroll
+      for (int w = 1; w < 8; ++w) a = ssq_merge(a, red[w][t]);
       if (r0 + t < o.R) o.part[(int64_t)blockIdx.y * o.R + r0 + t] = a;
     }
   }
+}
+
+// Pass 1b: one thread per row of A, then of B, merges the row's partials in
+// chunk order into its scale: q = T / f and 2^sh with norm = f 2^(e + en0).
+__global__ __launch_bounds__(256) void crt_scale_kernel(CrtOperand oa, CrtOperand ob) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool isa = g < oa.R;
+  const CrtOperand &o = isa ? oa : ob;
+  const int64_t r = isa ? g : g - oa.R;
+  if (r >= o.R) return;
+  Ssq a = Ssq{0.0, -2000, 0};
+  for (int c0 = 0; c0 < o.nkc; c0 += 8) {
+    Ssq b[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) b[c] = c0 + c < o.nkc ? o.part[(int64_t)(c0 + c) * o.R + r] : Ssq{0.0, -2000, 0};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) a = ssq_merge(a, b[c]);
+  }
+  RowScale rs{0.0, 1.0, 0, 0};
+  if (!(a.s <= DBL_MAX)) {
+    rs.iq = __builtin_nan("");
+  } else if (a.s > 0.0) {
+    int en0;
+    const double f = frexp(sqrt(a.s), &en0);   // f in [0.5, 1)
+    rs.q = o.Tm / f;
+    rs.iq = f / o.Tm;
+    rs.sh = (en0 + a.e) - o.eT;
+  }
+  o.scale[r] = rs;
 }
 
 // Pass 2: scale, round and reduce one 64-row x 64-K tile to one residue
@@ -191,29 +242,7 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(CrtOperand o, const Cr
   const int t = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
-  if (t < 64) {
-    const int64_t r = r0 + t;
-    double qv = 0.0, iq = 1.0;
-    int sh = 0;
-    if (r < o.R) {
-      Ssq a = o.part[r];
-      for (int cc = 1; cc < o.nkc; ++cc) a = ssq_merge(a, o.part[(int64_t)cc * o.R + r]);
-      if (!(a.s <= DBL_MAX)) {
-        iq = __builtin_nan("");
-      } else if (a.s > 0.0) {
-        int en0;
-        const double f = frexp(sqrt(a.s), &en0);   // norm = f 2^(en0 + e), f in [0.5, 1)
-        qv = o.Tm / f;
-        iq = f / o.Tm;
-        sh = o.eT - (en0 + a.e);
-      }
-      if (blockIdx.y == 0) o.scale[r] = RowScale{iq, -sh, 0};
-    }
-    sq[t] = qv;
-    ssh[t] = sh;
-  }
-  __syncthreads();
-  // 16 independent loads per thread, then the scaling.
+  // 16 independent loads per thread, issued before the row scales are read.
   double x[16];
   if (o.kfast) {
     // Lanes along K (tk = lane), rows wave + 4 it.
@@ -227,12 +256,6 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(CrtOperand o, const Cr
       x[it] = 0.0;
       if (kin && r < o.R) load_elem<double, false, true>(o.X, o.type, o.base + o.rowtab[r] + ko, x[it], im);
     }
-#pragma unroll
-    for (int it = 0; it < 16; ++it) {
-      const int tr = wave + 4 * it;
-      // |x 2^sh| <= f 2^eT < T: no overflow; qv = 0 zeroes empty and padding rows.
-      tile[tr][t & 63] = rint(ldexp(x[it], ssh[tr]) * sq[tr]);
-    }
   } else {
     // Lanes along the rows (tr = lane), K steps wave + 4 it.
     const int64_t r = r0 + (t & 63);
@@ -245,6 +268,22 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(CrtOperand o, const Cr
       x[it] = 0.0;
       if (rin && k < o.K) load_elem<double, false, true>(o.X, o.type, rbase + o.ktab[k], x[it], im);
     }
+  }
+  if (t < 64) {
+    // Rows past R (padding) get q = 0: their integers are zero.
+    const RowScale rs = r0 + t < o.R ? o.scale[r0 + t] : RowScale{0.0, 1.0, 0, 0};
+    sq[t] = rs.q;
+    ssh[t] = -rs.sh;
+  }
+  __syncthreads();
+  // |x 2^sh| <= f 2^eT < T: no overflow; q = 0 zeroes empty and padding rows.
+  if (o.kfast) {
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+      const int tr = wave + 4 * it;
+      tile[tr][t & 63] = rint(ldexp(x[it], ssh[tr]) * sq[tr]);
+    }
+  } else {
     const int tr = t & 63;
     const double qv = sq[tr];
     const int sh = ssh[tr];
@@ -290,11 +329,13 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(CrtOperand o, const Cr
 }
 
 // The residue GEMMs: one 256 x 256 tile of one modulus per workgroup, 8 waves
-// (2 x 4, each 128 x 64 = 8 x 4 MFMA blocks), K in stages of one fragment
-// (64 bytes): 16 A and 16 B row blocks, 32 KiB per stage, double-buffered in
-// LDS.  Two register sets alternate as the prefetch, so each stage's global
-// loads are issued two stages before they are written to LDS.  Every global
-// load, LDS write and LDS read is a contiguous 1 KiB per wave.
+// (2 x 4, each 128 x 64 = 8 x 4 MFMA blocks of v_mfma_i32_16x16x64_i8), K in
+// stages of one fragment (64 bytes).  A's 16 row blocks of a stage (16 KiB)
+// go through LDS, double-buffered, loaded two stages ahead; each wave reads
+// its 4 B fragments straight from global memory into registers one stage
+// ahead (a contiguous 1 KiB per wave in the residue layout, wanted by the 2
+// waves of a column, so L1 absorbs the repeat).  Measured against A and B
+// both through LDS: same time or better, half the LDS.
 struct CrtGemmArgs {
   const int8_t *ra, *rb;
   unsigned int *u;           // [Mp/16][Np/16][n][64]
@@ -315,12 +356,14 @@ __device__ __forceinline__ int crt_mod(int x, int m, float invm) {
 }
 
 // WM x WN waves, each TI x TJ blocks of 16 x 16: a (16 WM TI) x (16 WN TJ)
-// tile of one modulus per workgroup.
-template <int WM, int WN, int TI, int TJ, int MINB>
-__global__ __launch_bounds__(64 * WM * WN, MINB) void crt_gemm_kernel(const CrtGemmArgs a) {
-  constexpr int NW = WM * WN, FA = WM * TI, FB = WN * TJ, SLOTS = FA + FB, NLD = SLOTS / NW;
-  static_assert(SLOTS % NW == 0, "stage slots must split evenly over the waves");
-  __shared__ __attribute__((aligned(16))) int8_t smem[2][SLOTS * 1024];
+// tile of one modulus per workgroup.  B is loaded BS - 1 stages ahead into
+// BS rotating register sets (2 measured ahead of 4).
+template <int WM, int WN, int TI, int TJ, int BS>
+__global__ __launch_bounds__(64 * WM * WN, 1) void crt_gemm_kernel(const CrtGemmArgs a) {
+  constexpr int NW = WM * WN, FA = WM * TI, NLA = FA / NW;
+  static_assert(FA % NW == 0, "A stage slots must split evenly over the waves");
+  static_assert(BS == 2 || BS == 4, "B register sets: 2 or 4 (the loop is unrolled by 4)");
+  __shared__ __attribute__((aligned(16))) int8_t smem[2][FA * 1024];
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = wave % WM, wn = wave / WM;
@@ -328,30 +371,31 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void crt_gemm_kernel(const CrtG
   const int64_t bid = xcd_swizzle(blockIdx.x, (int64_t)gridDim.x);
   const int l = (int)(bid / tiles);
   const int64_t tt = bid % tiles;
-  // Groups of four tile rows walk the N tiles together (A panels stay in L2).
   const int64_t grp = tt / (4 * a.tiles_n), in = tt % (4 * a.tiles_n);
   const int64_t rows = min((int64_t)4, a.tiles_m - 4 * grp);
   const int64_t tm = 4 * grp + in % rows, tn = in / rows;
-  const int64_t mb0 = tm * FA, nb0 = tn * FB;
-  const int8_t *A = a.ra + ((int64_t)l * a.MB + mb0) * a.KB * 1024 + lane * 16;
-  const int8_t *B = a.rb + ((int64_t)l * a.NB + nb0) * a.KB * 1024 + lane * 16;
+  const int64_t mb0 = tm * FA, nb0 = tn * (WN * TJ);
+  const int64_t KB = a.KB;
+  const int8_t *A = a.ra + ((int64_t)l * a.MB + mb0) * KB * 1024 + lane * 16;
+  const int8_t *Bw = a.rb + ((int64_t)l * a.NB + nb0 + wn * TJ) * KB * 1024 + lane * 16;
   const int m = a.m[l];
   const float invm = a.invm[l];
-  const int64_t KB = a.KB;
 
-  // Stage slot q = wave + NW i (i < NLD): slots below FA are A row blocks,
-  // the rest B; slot q sits at q KiB of the LDS buffer.
-  auto load = [&](qs_v4i (&r)[NLD], int64_t kb) {
+  // Stage indices past the end are clamped to the last stage (loaded, never used).
+  auto clampk = [&](int64_t kb) { return kb < KB ? kb : KB - 1; };
+  auto loadA = [&](qs_v4i (&r)[NLA], int64_t kb) {
+    kb = clampk(kb);
 #pragma unroll
-    for (int i = 0; i < NLD; ++i) {
-      const int q = wave + NW * i;
-      const int8_t *p = q < FA ? A + ((int64_t)q * KB + kb) * 1024 : B + ((int64_t)(q - FA) * KB + kb) * 1024;
-      r[i] = *(const QS_GLOBAL qs_v4i *)p;
-    }
+    for (int i = 0; i < NLA; ++i) r[i] = *(const QS_GLOBAL qs_v4i *)(A + ((int64_t)(wave + NW * i) * KB + kb) * 1024);
   };
-  auto put = [&](const qs_v4i (&r)[NLD], int buf) {
+  auto putA = [&](const qs_v4i (&r)[NLA], int buf) {
 #pragma unroll
-    for (int i = 0; i < NLD; ++i) *(qs_v4i *)(smem[buf] + (wave + NW * i) * 1024 + lane * 16) = r[i];
+    for (int i = 0; i < NLA; ++i) *(qs_v4i *)(smem[buf] + (wave + NW * i) * 1024 + lane * 16) = r[i];
+  };
+  auto loadB = [&](qs_v4i (&r)[TJ], int64_t kb) {
+    kb = clampk(kb);
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) r[j] = *(const QS_GLOBAL qs_v4i *)(Bw + ((int64_t)j * KB + kb) * 1024);
   };
 
   qs_v4i acc[TI][TJ];
@@ -360,28 +404,47 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void crt_gemm_kernel(const CrtG
 #pragma unroll
     for (int j = 0; j < TJ; ++j) acc[i][j] = qs_v4i{0, 0, 0, 0};
 
-  // Iteration kb: refill `cur` (which held stage kb) with stage kb + 2,
-  // multiply stage kb out of LDS buffer kb & 1, then store `nxt` (stage
-  // kb + 1, loaded one iteration earlier) into the other buffer.  Loads and
-  // stores are unconditional (the last two stages are reloaded and stored
-  // unused) so the compiler's wait counts stay exact: each stage's loads
-  // have two iterations to land.
-  auto step = [&](int64_t kb, qs_v4i (&cur)[NLD], const qs_v4i (&nxt)[NLD]) {
-    load(cur, kb + 2 < KB ? kb + 2 : KB - 1);
-    __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the MFMAs
-    const int8_t *st = smem[kb & 1];
-    qs_v4i af[TI], bf[TJ];
+  qs_v4i as[2][NLA], bs[BS][TJ];
+  // Stage kb (U = kb mod 4): A stage kb + 2 into the A set stage kb left
+  // (already in LDS), B stage kb + BS - 1 into the B set stage kb - 1 left;
+  // multiply stage kb (A from LDS buffer kb & 1, B from set kb mod BS); A
+  // stage kb + 1 to the other LDS buffer.
+  auto step = [&](int64_t kb, auto U) {
+    constexpr int u = decltype(U)::value;
+    loadA(as[u & 1], kb + 2);
+    loadB(bs[(u + BS - 1) % BS], kb + BS - 1);
+    __builtin_amdgcn_sched_barrier(0);
+    const int8_t *st = smem[u & 1];
+    qs_v4i af[TI];
 #pragma unroll
     for (int i = 0; i < TI; ++i) af[i] = *(const qs_v4i *)(st + (wm * TI + i) * 1024 + lane * 16);
-#pragma unroll
-    for (int j = 0; j < TJ; ++j) bf[j] = *(const qs_v4i *)(st + (FA + wn * TJ + j) * 1024 + lane * 16);
 #pragma unroll
     for (int i = 0; i < TI; ++i)
 #pragma unroll
       for (int j = 0; j < TJ; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
-    if (((kb + 1) & 63) == 0 && kb + 1 < KB) {
-      // Every 4096 K: |acc| <= 4096 * 128^2 + m < 2^27 before, in [0, m) after.
+        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bs[u % BS][j], acc[i][j], 0, 0, 0);
+    putA(as[(u + 1) & 1], (u + 1) & 1);
+    __syncthreads();
+  };
+
+  loadA(as[0], 0);
+  loadA(as[1], 1);
+#pragma unroll
+  for (int q = 0; q < BS - 1; ++q) loadB(bs[q], q);
+  putA(as[0], 0);
+  __syncthreads();
+  // Chunks of 64 stages (4096 K) with no branch inside, so no load is moved
+  // past the MFMAs of its stage; the accumulators are reduced mod m between
+  // chunks (|acc| < 4096 * 128 * 128 = 2^26 within one).  KB is a multiple of 4.
+  for (int64_t kc = 0; kc < KB; kc += 64) {
+    const int64_t ke = kc + 64 < KB ? kc + 64 : KB;
+    for (int64_t kb = kc; kb < ke; kb += 4) {
+      step(kb, std::integral_constant<int, 0>());
+      step(kb + 1, std::integral_constant<int, 1>());
+      step(kb + 2, std::integral_constant<int, 2>());
+      step(kb + 3, std::integral_constant<int, 3>());
+    }
+    if (ke < KB) {
 #pragma unroll
       for (int i = 0; i < TI; ++i)
 #pragma unroll
@@ -389,21 +452,7 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void crt_gemm_kernel(const CrtG
 #pragma unroll
           for (int r = 0; r < 4; ++r) acc[i][j][r] = crt_mod(acc[i][j][r], m, invm);
     }
-    put(nxt, (int)((kb + 1) & 1));
-    __syncthreads();
-  };
-
-  qs_v4i r0[NLD], r1[NLD];
-  load(r0, 0);
-  load(r1, 1);
-  put(r0, 0);
-  __syncthreads();
-  for (int64_t kb = 0; kb < KB; kb += 2) {   // KB is even
-    step(kb, r0, r1);
-    step(kb + 1, r1, r0);
   }
-  // Lane l holds rows 4 (l / 16) + r, column l % 16 of each 16 x 16 block:
-  // one packed word per block, 256 contiguous bytes per wave.
 #pragma unroll
   for (int i = 0; i < TI; ++i)
 #pragma unroll
@@ -442,14 +491,22 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
   }
   const GemmArgs &g = a.g;
   const QS_GLOBAL unsigned int *u = (const QS_GLOBAL unsigned int *)a.u;
-  // Per block, all n residue words are loaded before any arithmetic.
+  // The residue words of two blocks are loaded before any arithmetic on them.
 #pragma unroll 1
-  for (int b = 0; b < 4; ++b) {
+  for (int b0 = 0; b0 < 4; b0 += 2) {
+  unsigned int wb[2][kMaxModuli];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int b = b0 + h;
     const int64_t blk = (MFAST ? mb0 + b : mb0) * a.NB + (MFAST ? nb0 : nb0 + b);
     const QS_GLOBAL unsigned int *up = u + blk * c.n * 64 + lane;
-    unsigned int w[kMaxModuli];
 #pragma unroll
-    for (int l = 0; l < kMaxModuli; ++l) w[l] = l < c.n ? up[l * 64] : 0u;
+    for (int l = 0; l < kMaxModuli; ++l) wb[h][l] = l < c.n ? up[l * 64] : 0u;
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int b = b0 + h;
+    const unsigned int (&w)[kMaxModuli] = wb[h];
     double s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int l = 0; l < kMaxModuli; ++l) {
@@ -471,6 +528,7 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
       else tile[wave][lr][16 * b + lc] = cp;
     }
   }
+  }
   __syncthreads();
   // Lane `lane` finishes 16 elements along C's unit-stride side: all 16
   // old-C loads are issued before the first store.
@@ -488,7 +546,7 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
       const int e = h + j;
       const int64_t other = (MFAST ? nb0 : mb0) * 16 + e;
       ok[j] = other < (MFAST ? g.N : g.M);
-      const RowScale os = ok[j] ? (MFAST ? a.sb[other] : a.sa[other]) : RowScale{1.0, 0, 0};
+      const RowScale os = ok[j] ? (MFAST ? a.sb[other] : a.sa[other]) : RowScale{0.0, 1.0, 0, 0};
       idx[j] = ok[j] ? g.c_base + foff + (MFAST ? g.nC[other] : g.mC[other]) : 0;
       v[j] = ldexp((MFAST ? tile[wave][lane][e] : tile[wave][e][lane]) * fs.iq * os.iq, fs.sh + os.sh);
       o[j] = 0.0;
@@ -695,7 +753,7 @@ static SplitLayout split_layout(const qs_plan *p) {
 qs_status split_prepare(qs_plan *p) {
   p->split_Mp = rup(p->M, kTile);
   p->split_Np = rup(p->N, kTile);
-  p->split_Kp = rup(std::max<int64_t>(p->K, 1), 128);   // an even number of 64-K stages
+  p->split_Kp = rup(std::max<int64_t>(p->K, 1), 256);   // a multiple of four 64-K stages (the K loop's unroll)
   const SplitLayout L = split_layout(p);
   if (hipMalloc(&p->d_split, L.total) != hipSuccess) return fail(QS_ERR_MEMORY, "plan_contract: split workspace");
   p->split_bytes = (int64_t)L.total;
@@ -741,11 +799,12 @@ hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t 
   ob.X = g.B; ob.type = g.b_type; ob.kfast = p->b_kfast; ob.base = g.b_base; ob.rowtab = g.nB; ob.ktab = g.kB;
   ob.R = p->N; ob.Rp = p->split_Np; ob.side = 1;
   ob.part = (Ssq *)(base + L.partB); ob.scale = (RowScale *)(base + L.scB); ob.res = (int8_t *)(base + L.rb);
-  for (const CrtOperand *o : {&oa, &ob}) {
-    hipLaunchKernelGGL(crt_norm_kernel, dim3((unsigned)((o->R + 63) / 64), (unsigned)nkc), dim3(256), 0, s, *o);
+  for (const CrtOperand *o : {&oa, &ob})
+    hipLaunchKernelGGL(crt_norm_kernel, dim3((unsigned)((o->R + 63) / 64), (unsigned)nkc), dim3(512), 0, s, *o);
+  hipLaunchKernelGGL(crt_scale_kernel, dim3((unsigned)((oa.R + ob.R + 255) / 256)), dim3(256), 0, s, oa, ob);
+  for (const CrtOperand *o : {&oa, &ob})
     hipLaunchKernelGGL(crt_residue_kernel, dim3((unsigned)(o->Rp / 64), (unsigned)(o->Kp / 64)), dim3(256), 0, s,
                        *o, c);
-  }
 
   CrtGemmArgs ga;
   ga.ra = oa.res;
@@ -763,8 +822,8 @@ hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t 
   // ahead of 128 x 128 (4 waves, 3 per CU) and 256 x 128 (8 waves, 2 per CU).
   ga.tiles_m = p->split_Mp / kTile;
   ga.tiles_n = p->split_Np / kTile;
-  hipLaunchKernelGGL((crt_gemm_kernel<2, 4, 8, 4, 1>), dim3((unsigned)(ga.tiles_m * ga.tiles_n * n)), dim3(512), 0, s,
-                     ga);
+  hipLaunchKernelGGL((crt_gemm_kernel<2, 4, 8, 4, 2>), dim3((unsigned)(ga.tiles_m * ga.tiles_n * n)), dim3(512), 0,
+                     s, ga);
 
   CrtRecArgs ra;
   ra.g = g;
