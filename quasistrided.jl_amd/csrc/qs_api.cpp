@@ -438,7 +438,16 @@ GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *
   g.tile = p->tile;
   g.a_kfast = p->a_kfast;
   g.b_kfast = p->b_kfast;
-  g.mixed = p->mixed;
+  // Spans of A and B as the tables address them; one storage type and spans
+  // within 32-bit byte offsets take the buffer-load gather.
+  g.a_lo = a_base + p->mA_lo + p->kA_lo;
+  g.b_lo = b_base + p->kB_lo + p->nB_lo;
+  const int64_t a_bytes = (p->mA_hi - p->mA_lo + p->kA_hi - p->kA_lo + 1) * elem_bytes(d.a_type);
+  const int64_t b_bytes = (p->kB_hi - p->kB_lo + p->nB_hi - p->nB_lo + 1) * elem_bytes(d.b_type);
+  const bool buf = !p->mixed && a_bytes <= kBufferSpanMax && b_bytes <= kBufferSpanMax;
+  g.a_bytes = buf ? (uint32_t)a_bytes : 0;
+  g.b_bytes = buf ? (uint32_t)b_bytes : 0;
+  g.mixed = buf ? 0 : 1;
   return g;
 }
 

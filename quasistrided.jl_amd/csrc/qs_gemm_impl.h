@@ -234,7 +234,7 @@ __device__ const double qs_zero16[2] = {0.0, 0.0};
 // t/BK + r*(256/BK).  Row offsets are read once; K offsets are fetched one
 // stage ahead of the data that uses them.  Loads are unconditional at
 // clamped in-range coordinates; out-of-tile values become zero.
-template <typename R, bool CPLX, bool MIXED, int ROWS, int BK, bool KF>
+template <typename R, bool CPLX, bool MIXED, int ROWS, int BK, bool KF, int VS = 2>
 struct Gather {
   static constexpr int P = CPLX ? 2 : 1;
   static constexpr int NL = (ROWS * BK + 255) / 256;
@@ -244,48 +244,88 @@ struct Gather {
   static constexpr int KSTEP = KF ? 0 : 256 / ROWS;
   static constexpr bool UNIFORM_K = !KF && ROWS >= 64;
   static constexpr bool PARTIAL = ROWS * BK % 256 != 0;
-  int64_t rowoff[NR];
+  // One storage type: buffer loads over the operand's accessed span, with
+  // 32-bit byte offsets; an element outside the tile reads past the span,
+  // which the hardware returns as zero.  Mixed types: 64-bit element
+  // indices and a pointer select (the device zero for outside elements).
+  static constexpr bool BUF = !MIXED;
+  static constexpr int ESZ = P * (int)sizeof(R);
+  static constexpr uint32_t OOB = 0xFFFFFFF0u;
+  typedef typename std::conditional<BUF, uint32_t, int64_t>::type Off;
+  // K offsets of VS stages (set = stage mod VS).  Wave-uniform offsets are
+  // scalar loads, outside the in-order vector-load count: one set, fetched
+  // after the loads that use it.
+  static constexpr int KS = UNIFORM_K ? 1 : VS;
+  __amdgpu_buffer_rsrc_t rsrc;
+  Off rowoff[NR];
   bool rowok[NR];
   int row0, kk0;
-  int64_t koff[NK];
-  bool kok[NK];
-  R val[2][NL][P];   // two stages in flight
+  Off koff[KS][NK];
+  bool kok[KS][NK];
+  R val[VS][NL][P];   // VS stages in flight
 
+  // `lo` / `bytes`: the operand's accessed span (elements from its storage
+  // start, and its length in bytes) for the buffer loads.
   __device__ __forceinline__ void init(int t, int64_t r0, int64_t rows_total, int64_t base,
-                                       const int64_t *__restrict__ rowtab) {
+                                       const int64_t *__restrict__ rowtab, const void *p, int64_t lo, uint32_t bytes) {
     row0 = KF ? t / BK : t % ROWS;
     kk0 = KF ? t % BK : t / ROWS;
     if (UNIFORM_K) kk0 = __builtin_amdgcn_readfirstlane(kk0);
+    if constexpr (BUF)
+      rsrc = __builtin_amdgcn_make_buffer_rsrc((void *)((const char *)p + lo * ESZ), (short)0, (int)bytes, 0x00020000);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       const int64_t gi = r0 + row0 + r * RSTEP;
       rowok[r] = gi < rows_total && (!PARTIAL || t + 256 * r < ROWS * BK);
-      rowoff[r] = base + rowtab[gi < rows_total ? gi : rows_total - 1];
+      const int64_t o = base + rowtab[gi < rows_total ? gi : rows_total - 1];
+      // Row and K byte offsets add up, modulo 2^32, to the element's offset in the span.
+      rowoff[r] = BUF ? (Off)(uint32_t)((o - lo) * ESZ) : (Off)o;
     }
   }
+  template <int S>
   __device__ __forceinline__ void fetch_k(int64_t k0, int64_t kend, const int64_t *__restrict__ ktab) {
 #pragma unroll
     for (int r = 0; r < NK; ++r) {
       const int64_t gk = k0 + kk0 + r * KSTEP;
-      kok[r] = gk < kend;
+      kok[S % KS][r] = gk < kend;
       const int64_t gkc = gk < kend ? gk : kend - 1;
-      koff[r] = UNIFORM_K ? ((qs_const_i64)ktab)[gkc] : ktab[gkc];
+      const int64_t o = UNIFORM_K ? ((qs_const_i64)ktab)[gkc] : ktab[gkc];
+      koff[S % KS][r] = BUF ? (Off)(uint32_t)(o * ESZ) : (Off)o;
     }
   }
-  // Issue the stage's loads.  An element outside the tile reads the device
-  // zero instead (a select on the address, not on the value), so nothing
-  // waits on the loads until `stage` consumes them.
+  // Issue the stage's loads; nothing waits on them until `stage` consumes them.
   template <int S>
   __device__ __forceinline__ void load(const void *__restrict__ p, int type) {
 #pragma unroll
     for (int r = 0; r < NL; ++r) {
       const int ir = KF ? r : 0, kr = KF ? 0 : r;
-      const bool ok = rowok[ir] && kok[kr];
-      const void *src = ok ? p : (const void *)qs_zero16;
-      const int64_t idx = ok ? rowoff[ir] + koff[kr] : 0;
-      R im;
-      load_elem<R, CPLX, MIXED>(src, type, idx, val[S][r][0], im);
-      if (CPLX) val[S][r][P - 1] = im;
+      const bool ok = rowok[ir] && kok[S % KS][kr];
+      if constexpr (BUF) {
+        const uint32_t off = ok ? rowoff[ir] + koff[S % KS][kr] : OOB;
+        // The loaded vector is cast whole to a scalar or a struct before any
+        // part is taken: extracting its lanes directly lets the compiler
+        // narrow the load to one dword (and then read the wrong lane).
+        if constexpr (ESZ == 4) {
+          val[S][r][0] = __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0));
+        } else if constexpr (ESZ == 8 && !CPLX) {
+          val[S][r][0] = __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
+        } else if constexpr (ESZ == 8) {
+          const uint64_t u = __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
+          val[S][r][0] = __builtin_bit_cast(R, (uint32_t)u);
+          val[S][r][P - 1] = __builtin_bit_cast(R, (uint32_t)(u >> 32));
+        } else {
+          struct Pair { R re, im; };
+          const Pair v = __builtin_bit_cast(Pair, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+          val[S][r][0] = v.re;
+          val[S][r][P - 1] = v.im;
+        }
+      } else {
+        const void *src = ok ? p : (const void *)qs_zero16;
+        const int64_t idx = ok ? rowoff[ir] + koff[S % KS][kr] : 0;
+        R im;
+        load_elem<R, CPLX, MIXED>(src, type, idx, val[S][r][0], im);
+        if (CPLX) val[S][r][P - 1] = im;
+      }
     }
   }
   template <int S, int PLANE>
@@ -333,10 +373,6 @@ struct TileGemm {
     const int64_t kbeg = (int64_t)split * g.k_chunk;
     const int64_t kend = min(g.K, kbeg + g.k_chunk);
 
-    Gather<R, CPLX, MIXED, BM, BK, AKF> ga;
-    Gather<R, CPLX, MIXED, BN, BK, BKF> gb;
-    ga.init(t, m0, g.M, g.a_base, g.mA);
-    gb.init(t, n0, g.N, g.b_base, g.nB);
     const R sa = g.conj_a ? (R)-1 : (R)1, sb = g.conj_b ? (R)-1 : (R)1;
 
     V acc[P][FM][FN];
@@ -349,83 +385,134 @@ struct TileGemm {
 
     const int fr = lane & 15, fk = lane >> 4;
     if constexpr (DEEP) {
-      // Stage j of the K range lives in LDS buffer j & 1 and, while in flight,
-      // in gather register set j & 1.  Iteration j issues the loads of stage
-      // j + 2 (unconditionally: past kend every lane reads the device zero, so
-      // the compiler's wait counts stay exact), multiplies stage j and stores
-      // stage j + 1 into LDS: each stage's loads have two iterations to land.
-      auto step = [&](int64_t k0, auto cur, auto nxt) {
-        constexpr int C = decltype(cur)::value, N = decltype(nxt)::value;
-        ga.template load<C>(g.A, g.a_type);
-        gb.template load<C>(g.B, g.b_type);
-        ga.fetch_k(k0 + 3 * BK, kend, g.kA);
-        gb.fetch_k(k0 + 3 * BK, kend, g.kB);
-        __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the MFMAs
-        const R *As = smem + C * (SA + SB);
-        const R *Bs = As + SA;
+      // NS gather register sets (2, or 4 for tiles whose gathers are small).
+      // Stage j of the K range lives in LDS buffer j & 1 and, while in
+      // flight, in gather register set j mod NS (its K offsets in offset set
+      // j mod NS).  Iteration j issues the loads of stage j + NS
+      // (unconditionally: past kend every lane reads the device zero, so the
+      // compiler's wait counts stay exact), multiplies stage j and stores
+      // stage j + 1 into LDS: each stage's loads have NS - 1 iterations and a
+      // part to land.  Per-lane K offsets are fetched NS - 1 iterations
+      // before their loads (stage j + 2 NS - 1 in iteration j) and issued
+      // ahead of the iteration's loads: vector loads complete in order, so
+      // the wait for an offset covers every load issued before it, and this
+      // way it never reaches a load younger than the one being staged.
+      // Wave-uniform offsets (scalar loads) are fetched one stage ahead.
+      auto deep = [&](auto ns) {
+        constexpr int NS = decltype(ns)::value;
+        Gather<R, CPLX, MIXED, BM, BK, AKF, NS> ga;
+        Gather<R, CPLX, MIXED, BN, BK, BKF, NS> gb;
+        ga.init(t, m0, g.M, g.a_base, g.mA, g.A, g.a_lo, g.a_bytes);
+        gb.init(t, n0, g.N, g.b_base, g.nB, g.B, g.b_lo, g.b_bytes);
+        typedef decltype(ga) GA;
+        typedef decltype(gb) GB;
+        auto step = [&](int64_t k0, auto u) {
+          constexpr int U = decltype(u)::value, NX = (U + 1) % NS, KL = (U + NS - 1) % NS;
+          if constexpr (!GA::UNIFORM_K) ga.template fetch_k<KL>(k0 + (2 * NS - 1) * BK, kend, g.kA);
+          if constexpr (!GB::UNIFORM_K) gb.template fetch_k<KL>(k0 + (2 * NS - 1) * BK, kend, g.kB);
+          __builtin_amdgcn_sched_barrier(0);   // offsets issued before the loads
+          ga.template load<U>(g.A, g.a_type);
+          gb.template load<U>(g.B, g.b_type);
+          if constexpr (GA::UNIFORM_K) ga.template fetch_k<0>(k0 + (NS + 1) * BK, kend, g.kA);
+          if constexpr (GB::UNIFORM_K) gb.template fetch_k<0>(k0 + (NS + 1) * BK, kend, g.kB);
+          __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the MFMAs
+          const R *As = smem + (U & 1) * (SA + SB);
+          const R *Bs = As + SA;
   #pragma unroll
-        for (int ks = 0; ks < BK / 4; ++ks) {
-          R af[P][FM], bf[P][FN];
+          for (int ks = 0; ks < BK / 4; ++ks) {
+            R af[P][FM], bf[P][FN];
   #pragma unroll
-          for (int p = 0; p < P; ++p) {
+            for (int p = 0; p < P; ++p) {
   #pragma unroll
-            for (int a = 0; a < FM; ++a) af[p][a] = As[p * LA::plane + LA::at(wm * WTM + a * 16 + fr, ks * 4 + fk)];
+              for (int a = 0; a < FM; ++a) af[p][a] = As[p * LA::plane + LA::at(wm * WTM + a * 16 + fr, ks * 4 + fk)];
   #pragma unroll
-            for (int b = 0; b < FN; ++b) bf[p][b] = Bs[p * LB::plane + LB::at(wn * WTN + b * 16 + fr, ks * 4 + fk)];
-          }
-  #pragma unroll
-          for (int a = 0; a < FM; ++a)
-  #pragma unroll
-            for (int b = 0; b < FN; ++b) {
-              if (!CPLX) {
-                acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
-              } else {
-                acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
-                acc[0][a][b] = mfma16(-af[P - 1][a], bf[P - 1][b], acc[0][a][b]);
-                acc[P - 1][a][b] = mfma16(af[0][a], bf[P - 1][b], acc[P - 1][a][b]);
-                acc[P - 1][a][b] = mfma16(af[P - 1][a], bf[0][b], acc[P - 1][a][b]);
-              }
+              for (int b = 0; b < FN; ++b) bf[p][b] = Bs[p * LB::plane + LB::at(wn * WTN + b * 16 + fr, ks * 4 + fk)];
             }
+  #pragma unroll
+            for (int a = 0; a < FM; ++a)
+  #pragma unroll
+              for (int b = 0; b < FN; ++b) {
+                if (!CPLX) {
+                  acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+                } else {
+                  acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+                  acc[0][a][b] = mfma16(-af[P - 1][a], bf[P - 1][b], acc[0][a][b]);
+                  acc[P - 1][a][b] = mfma16(af[0][a], bf[P - 1][b], acc[P - 1][a][b]);
+                  acc[P - 1][a][b] = mfma16(af[P - 1][a], bf[0][b], acc[P - 1][a][b]);
+                }
+              }
+          }
+          R *Ws = smem + ((U + 1) & 1) * (SA + SB);
+          ga.template stage<NX, LA::plane>(Ws, t, sa);
+          gb.template stage<NX, LB::plane>(Ws + SA, t, sb);
+          __syncthreads();
+        };
+        if (kbeg < kend) {
+          // The same issue order as the loop ([per-lane offsets of stage
+          // s + NS - 1][loads of stage s]), so the wait counts agree on entry
+          // and on the back edge.
+          ga.template fetch_k<0>(kbeg, kend, g.kA);
+          gb.template fetch_k<0>(kbeg, kend, g.kB);
+          if constexpr (NS == 4) {
+            if constexpr (!GA::UNIFORM_K) {
+              ga.template fetch_k<1>(kbeg + BK, kend, g.kA);
+              ga.template fetch_k<2>(kbeg + 2 * BK, kend, g.kA);
+            }
+            if constexpr (!GB::UNIFORM_K) {
+              gb.template fetch_k<1>(kbeg + BK, kend, g.kB);
+              gb.template fetch_k<2>(kbeg + 2 * BK, kend, g.kB);
+            }
+          }
+          auto pro = [&](auto sc) {
+            constexpr int S = decltype(sc)::value;
+            if constexpr (!GA::UNIFORM_K) ga.template fetch_k<(S + NS - 1) % NS>(kbeg + (S + NS - 1) * BK, kend, g.kA);
+            if constexpr (!GB::UNIFORM_K) gb.template fetch_k<(S + NS - 1) % NS>(kbeg + (S + NS - 1) * BK, kend, g.kB);
+            __builtin_amdgcn_sched_barrier(0);
+            ga.template load<S>(g.A, g.a_type);
+            gb.template load<S>(g.B, g.b_type);
+            if constexpr (GA::UNIFORM_K) ga.template fetch_k<0>(kbeg + (S + 1) * BK, kend, g.kA);
+            if constexpr (GB::UNIFORM_K) gb.template fetch_k<0>(kbeg + (S + 1) * BK, kend, g.kB);
+            __builtin_amdgcn_sched_barrier(0);
+          };
+          pro(std::integral_constant<int, 0>());
+          pro(std::integral_constant<int, 1>());
+          if constexpr (NS == 4) {
+            pro(std::integral_constant<int, 2>());
+            pro(std::integral_constant<int, 3>());
+          }
+          ga.template stage<0, LA::plane>(smem, t, sa);
+          gb.template stage<0, LB::plane>(smem + SA, t, sb);
         }
-        R *Ws = smem + N * (SA + SB);
-        ga.template stage<N, LA::plane>(Ws, t, sa);
-        gb.template stage<N, LB::plane>(Ws + SA, t, sb);
         __syncthreads();
+        // NS stages per trip, all unconditional: stages past kend are all
+        // zeros (every lane past kend reads the device zero).
+        for (int64_t k0 = kbeg; k0 < kend; k0 += NS * BK) {
+          step(k0, std::integral_constant<int, 0>());
+          step(k0 + BK, std::integral_constant<int, 1>());
+          if constexpr (NS == 4) {
+            step(k0 + 2 * BK, std::integral_constant<int, 2>());
+            step(k0 + 3 * BK, std::integral_constant<int, 3>());
+          }
+        }
       };
-      if (kbeg < kend) {
-        ga.fetch_k(kbeg, kend, g.kA);
-        gb.fetch_k(kbeg, kend, g.kB);
-        ga.template load<0>(g.A, g.a_type);
-        gb.template load<0>(g.B, g.b_type);
-        ga.fetch_k(kbeg + BK, kend, g.kA);
-        gb.fetch_k(kbeg + BK, kend, g.kB);
-        ga.template load<1>(g.A, g.a_type);
-        gb.template load<1>(g.B, g.b_type);
-        ga.fetch_k(kbeg + 2 * BK, kend, g.kA);
-        gb.fetch_k(kbeg + 2 * BK, kend, g.kB);
-        ga.template stage<0, LA::plane>(smem, t, sa);
-        gb.template stage<0, LB::plane>(smem + SA, t, sb);
-      }
-      __syncthreads();
-      typedef std::integral_constant<int, 0> S0;
-      typedef std::integral_constant<int, 1> S1;
-      // Two stages per trip, both unconditional: with an odd stage count the
-      // last one is all zeros (every lane past kend reads the device zero).
-      for (int64_t k0 = kbeg; k0 < kend; k0 += 2 * BK) {
-        step(k0, S0(), S1());
-        step(k0 + BK, S1(), S0());
-      }
+      // Two sets: four (loads four stages ahead) measured slower on the
+      // 32 x 64 FP64 tile, 75 against 71 us at M = N = K = 1024.
+      deep(std::integral_constant<int, 2>());
     } else {
       // One stage in flight: stage j + 1 is loaded while stage j multiplies.
+      Gather<R, CPLX, MIXED, BM, BK, AKF> ga;
+      Gather<R, CPLX, MIXED, BN, BK, BKF> gb;
+      ga.init(t, m0, g.M, g.a_base, g.mA, g.A, g.a_lo, g.a_bytes);
+      gb.init(t, n0, g.N, g.b_base, g.nB, g.B, g.b_lo, g.b_bytes);
       int buf = 0;
       if (kbeg < kend) {
-        ga.fetch_k(kbeg, kend, g.kA);
-        gb.fetch_k(kbeg, kend, g.kB);
+        ga.template fetch_k<0>(kbeg, kend, g.kA);
+        gb.template fetch_k<0>(kbeg, kend, g.kB);
         ga.template load<0>(g.A, g.a_type);
         gb.template load<0>(g.B, g.b_type);
         if (kbeg + BK < kend) {
-          ga.fetch_k(kbeg + BK, kend, g.kA);
-          gb.fetch_k(kbeg + BK, kend, g.kB);
+          ga.template fetch_k<0>(kbeg + BK, kend, g.kA);
+          gb.template fetch_k<0>(kbeg + BK, kend, g.kB);
         }
         ga.template stage<0, LA::plane>(smem, t, sa);
         gb.template stage<0, LB::plane>(smem + SA, t, sb);
@@ -437,8 +524,8 @@ struct TileGemm {
           ga.template load<0>(g.A, g.a_type);
           gb.template load<0>(g.B, g.b_type);
           if (k0 + 2 * BK < kend) {
-            ga.fetch_k(k0 + 2 * BK, kend, g.kA);
-            gb.fetch_k(k0 + 2 * BK, kend, g.kB);
+            ga.template fetch_k<0>(k0 + 2 * BK, kend, g.kA);
+            gb.template fetch_k<0>(k0 + 2 * BK, kend, g.kB);
           }
         }
         const R *As = smem + buf * (SA + SB);

@@ -181,7 +181,7 @@ static qs_status build_tables(qs_group_plan *g) {
     GroupItem it;
     std::memset(&it, 0, sizeof it);
     it.g = a;
-    if (p->mixed) {   // mixed storage types run standalone, on the same stream
+    if (a.mixed) {   // the generic gather (mixed types, spans past 4 GB) runs standalone, on the same stream
       g->standalone.push_back(p);
       continue;
     }

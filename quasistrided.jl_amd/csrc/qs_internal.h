@@ -50,9 +50,16 @@ struct GemmArgs {
   void *partials;         // [k_splits][planes][M][N] of the real compute type
   int tile;               // TileId
   int a_kfast, b_kfast;   // gather lanes along K (1) or along M / N (0)
-  int mixed;              // some storage type differs from the compute type
+  int mixed;              // the generic gather: some storage type differs from the compute
+                          // type, or an operand span exceeds the 32-bit buffer range
   int *tile_counters;     // k_splits > 1: per-tile arrivals, zero between launches
+  int64_t a_lo, b_lo;     // accessed spans of A and B (elements from the storage start) ...
+  uint32_t a_bytes, b_bytes;   // ... and their lengths in bytes, for the buffer-load gather
 };
+
+// Largest operand span, in bytes, that the buffer-load gather addresses
+// (32-bit offsets, with room above it for the out-of-range offset).
+constexpr int64_t kBufferSpanMax = 0xFFFFFF00LL;
 
 // Launch the tiled MFMA GEMM (with k_splits > 1, the last split of each tile
 // reduces the partials in a fixed order).  Shapes and tables must already be validated.

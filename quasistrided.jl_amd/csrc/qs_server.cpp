@@ -128,7 +128,7 @@ static void worker_loop(qs_server *s) {
       std::vector<Request *> solo;
       for (uint64_t id : batch) {
         Request &r = s->reqs[id];
-        if (!s->grouped || r.plan->mixed) {
+        if (!s->grouped || r.g.mixed) {
           solo.push_back(&r);
           continue;
         }
