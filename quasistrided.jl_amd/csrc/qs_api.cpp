@@ -177,7 +177,7 @@ static int choose_splits(int64_t tiles, int64_t K, int bk, int64_t &k_chunk) {
     s = std::max(s, 1);
     s = std::min(s, 16);
   }
-  k_chunk = cdiv(cdiv(K, s), bk) * bk;
+  k_chunk = cdiv(cdiv(K, s), 2 * bk) * 2 * bk;   // whole stage pairs (the gather's loop step)
   s = (int)cdiv(K, k_chunk);
   return s;
 }
@@ -277,13 +277,21 @@ qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
     p->b_kfast = bkf;
   }
 
-  // Offset tables, built once and kept on the device.
-  const int64_t nt = 2 * (M + N + K);
+  // Offset tables, built once and kept on the device.  The K tables carry
+  // kKTablePad entries past K whose offsets put any A (B) row 2^31 bytes past
+  // the start of its span: the buffer-load gather reads them as zero, with
+  // no bounds test (its spans stay below 2^31 bytes).
+  const int64_t KP = K + kKTablePad;
+  const int64_t nt = 2 * (M + N + KP);
   std::vector<int64_t> h((size_t)std::max<int64_t>(nt, 1));
-  int64_t *mA = h.data(), *mC = mA + M, *nB = mC + M, *nC = nB + N, *kA = nC + N, *kB = kA + K;
+  int64_t *mA = h.data(), *mC = mA + M, *nB = mC + M, *nC = nB + N, *kA = nC + N, *kB = kA + KP;
   fill_offsets(mg, 0, M, mA, mC);
   fill_offsets(ng, 0, N, nB, nC);
   fill_offsets(kg, 0, K, kA, kB);
+  for (int64_t k = K; k < KP; ++k) {
+    kA[k] = p->kA_lo + (int64_t(1) << 31) / elem_bytes(d.a_type);
+    kB[k] = p->kB_lo + (int64_t(1) << 31) / elem_bytes(d.b_type);
+  }
   p->c_mfast = (M > 1 && mC[1] - mC[0] == 1) || !(N > 1 && nC[1] - nC[0] == 1);
   hipError_t e = hipSuccess;
   if (nt > 0) {
@@ -300,7 +308,7 @@ qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
   p->t_nB = p->t_mC + M;
   p->t_nC = p->t_nB + N;
   p->t_kA = p->t_nC + N;
-  p->t_kB = p->t_kA + K;
+  p->t_kB = p->t_kA + KP;
 
   // Tile, splits, workspace: an explicit override, else the autotune cache's
   // choice for this shape class, else the heuristic.
@@ -337,7 +345,7 @@ qs_status configure_gemm(qs_plan *p, int tile, int splits) {
   if (splits <= 0) {
     ks = choose_splits(tiles, p->K, ts.bk, chunk);
   } else {
-    chunk = cdiv(cdiv(p->K, splits), ts.bk) * ts.bk;
+    chunk = cdiv(cdiv(p->K, splits), 2 * ts.bk) * 2 * ts.bk;
     ks = (int)cdiv(p->K, chunk);
   }
   // Partial sums, then one arrival counter per tile (256-byte aligned).

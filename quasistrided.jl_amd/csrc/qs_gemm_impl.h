@@ -185,7 +185,7 @@ __device__ __forceinline__ void store_c(const GemmArgs &g, int64_t row, int64_t 
 
 // K depth of one LDS stage: the largest power of two in [4, 32] whose double-
 // buffered A and B tiles fit the LDS budget (76 KiB: two workgroups per CU).
-constexpr int lds_pitch_rowmajor(int rsize, int bk) { return rsize == 8 ? bk + 2 : ((bk / 4) % 2 ? bk : bk + 4); }
+constexpr int lds_pitch_rowmajor(int rsize, int bk) { return rsize == 8 ? bk + 1 : ((bk / 4) % 2 ? bk : bk + 4); }
 constexpr int pick_bk_for(int rsize, int planes, int bm, int bn, bool akf, bool bkf) {
   int best = 4;
   for (int bk = 4; bk <= 32; bk *= 2) {
@@ -245,12 +245,15 @@ struct Gather {
   static constexpr bool UNIFORM_K = !KF && ROWS >= 64;
   static constexpr bool PARTIAL = ROWS * BK % 256 != 0;
   // One storage type: buffer loads over the operand's accessed span, with
-  // 32-bit byte offsets; an element outside the tile reads past the span,
-  // which the hardware returns as zero.  Mixed types: 64-bit element
-  // indices and a pointer select (the device zero for outside elements).
+  // 32-bit byte offsets and no bounds tests.  Rows past M (N) repeat the last
+  // row: they only reach accumulators of rows (columns) that are never
+  // stored.  K past the range reads the K tables' pad entries, whose offsets
+  // land past the span, which the hardware returns as zero.  A split's K
+  // range is a whole number of stage pairs, so no multiplied stage reaches
+  // into the next split.  Mixed types: 64-bit element indices, bounds tests
+  // and a pointer select (the device zero for outside elements).
   static constexpr bool BUF = !MIXED;
   static constexpr int ESZ = P * (int)sizeof(R);
-  static constexpr uint32_t OOB = 0xFFFFFFF0u;
   typedef typename std::conditional<BUF, uint32_t, int64_t>::type Off;
   // K offsets of VS stages (set = stage mod VS).  Wave-uniform offsets are
   // scalar loads, outside the in-order vector-load count: one set, fetched
@@ -287,10 +290,14 @@ struct Gather {
 #pragma unroll
     for (int r = 0; r < NK; ++r) {
       const int64_t gk = k0 + kk0 + r * KSTEP;
-      kok[S % KS][r] = gk < kend;
-      const int64_t gkc = gk < kend ? gk : kend - 1;
-      const int64_t o = UNIFORM_K ? ((qs_const_i64)ktab)[gkc] : ktab[gkc];
-      koff[S % KS][r] = BUF ? (Off)(uint32_t)(o * ESZ) : (Off)o;
+      if constexpr (BUF) {
+        const int64_t o = UNIFORM_K ? ((qs_const_i64)ktab)[gk] : ktab[gk];
+        koff[S % KS][r] = (Off)((uint32_t)o * (uint32_t)ESZ);
+      } else {
+        kok[S % KS][r] = gk < kend;
+        const int64_t gkc = gk < kend ? gk : kend - 1;
+        koff[S % KS][r] = UNIFORM_K ? ((qs_const_i64)ktab)[gkc] : ktab[gkc];
+      }
     }
   }
   // Issue the stage's loads; nothing waits on them until `stage` consumes them.
@@ -299,9 +306,8 @@ struct Gather {
 #pragma unroll
     for (int r = 0; r < NL; ++r) {
       const int ir = KF ? r : 0, kr = KF ? 0 : r;
-      const bool ok = rowok[ir] && kok[S % KS][kr];
       if constexpr (BUF) {
-        const uint32_t off = ok ? rowoff[ir] + koff[S % KS][kr] : OOB;
+        const uint32_t off = rowoff[ir] + koff[S % KS][kr];
         // The loaded vector is cast whole to a scalar or a struct before any
         // part is taken: extracting its lanes directly lets the compiler
         // narrow the load to one dword (and then read the wrong lane).
@@ -320,6 +326,7 @@ struct Gather {
           val[S][r][P - 1] = v.im;
         }
       } else {
+        const bool ok = rowok[ir] && kok[S % KS][kr];
         const void *src = ok ? p : (const void *)qs_zero16;
         const int64_t idx = ok ? rowoff[ir] + koff[S % KS][kr] : 0;
         R im;

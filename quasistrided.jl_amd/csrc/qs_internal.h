@@ -46,7 +46,7 @@ struct GemmArgs {
   double alpha_re, alpha_im, beta_re, beta_im;
   int beta_case;          // 0: beta == 0, 1: beta == 1, 2: general
   int k_splits;           // > 1: partial sums into `partials`, reduced in a fixed order
-  int64_t k_chunk;        // K steps per split (multiple of the tile's BK)
+  int64_t k_chunk;        // K steps per split (a multiple of twice the tile's BK)
   void *partials;         // [k_splits][planes][M][N] of the real compute type
   int tile;               // TileId
   int a_kfast, b_kfast;   // gather lanes along K (1) or along M / N (0)
@@ -57,9 +57,12 @@ struct GemmArgs {
   uint32_t a_bytes, b_bytes;   // ... and their lengths in bytes, for the buffer-load gather
 };
 
-// Largest operand span, in bytes, that the buffer-load gather addresses
-// (32-bit offsets, with room above it for the out-of-range offset).
-constexpr int64_t kBufferSpanMax = 0xFFFFFF00LL;
+// Largest operand span, in bytes, that the buffer-load gather addresses: a
+// K-table pad entry moves an offset 2^31 bytes on, past any such span.
+constexpr int64_t kBufferSpanMax = 0x7FFFFF00LL;
+// Entries past K in the K offset tables: the gather prefetches up to five
+// stages of at most 32 K beyond the last one it multiplies.
+constexpr int64_t kKTablePad = 192;
 
 // Launch the tiled MFMA GEMM (with k_splits > 1, the last split of each tile
 // reduces the partials in a fixed order).  Shapes and tables must already be validated.
