@@ -322,6 +322,66 @@ def test_split_k(oracle, dtype):
 
 
 @pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+def test_forced_split_k_ragged(oracle, tile):
+    """Split-K chunks are whole stage pairs; the last split ends at a ragged K
+    (the K tables' pad entries read as zero)."""
+    case(f"forced split-K 3, ragged K=1000, tile {tile}")
+    rng = np.random.default_rng(40 + tile)
+    M, K, N = 100, 1000, 70
+    for dtype in (np.float64, np.complex128):
+        Av = dev_view(randn(rng, dtype, M * K), (M, K), (1, M))
+        Bv = dev_view(randn(rng, dtype, K * N), (K, N), (N, 1) if tile % 2 else (1, K))
+        Cv = dev_view(randn(rng, dtype, M * N), (M, N), (1, M))
+        plan = run_and_check(oracle, Cv, Av, (1, 2), Bv, (2, 3), (1, 3), 1.0, 0.5,
+                             kernel=qs.MFMAKernel(tile=tile, k_splits=3), path_modes=qs.PathModes("never", "never"))
+        assert plan.info["k_splits"] == 3, plan
+
+
+def test_edge_rows_do_not_leak():
+    """Rows past M (columns past N) in a partial tile repeat the last one; a
+    NaN and an Inf there reach only that row (column) of C."""
+    case("NaN / Inf in the last row of A and the last column of B, partial tiles")
+    rng = np.random.default_rng(44)
+    M, K, N = 70, 96, 50
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    A[M - 1, 5] = np.nan
+    B[7, N - 1] = np.inf
+    Av = dev_view(A.reshape(-1, order="F"), (M, K), (1, M))
+    Bv = dev_view(B.reshape(-1, order="F"), (K, N), (1, K))
+    Cv = dev_view(np.zeros(M * N), (M, N), (1, M))
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), path_modes=qs.PathModes("never", "never"))
+    qs.execute(plan, 1.0, 0.0)
+    torch.cuda.synchronize()
+    got = values(Cv)
+    want = A[: M - 1] @ B[:, : N - 1]
+    assert rel_fro(got[: M - 1, : N - 1], want) <= 1e-13
+    assert np.all(np.isnan(got[M - 1]))
+    assert not np.any(np.isfinite(got[: M - 1, N - 1]))
+
+
+def test_span_past_2gib_takes_generic_gather():
+    """An operand whose accessed span exceeds the buffer-load range (2^31
+    bytes) runs the generic 64-bit gather, with the same results."""
+    case("A span of 2.6 GB (stride 2^23 along K), F64")
+    rng = np.random.default_rng(45)
+    M, K, N, ks = 64, 40, 32, 1 << 23
+    st = torch.zeros((K - 1) * ks + M, dtype=torch.float64, device=DEV)
+    A = rng.standard_normal((M, K))
+    idx = (np.arange(M)[:, None] + ks * np.arange(K)[None, :]).reshape(-1)
+    st[torch.from_numpy(idx).to(DEV)] = torch.from_numpy(A.reshape(-1)).to(DEV)
+    Av = SV(st, (M, K), (1, ks), 0)
+    B = rng.standard_normal((K, N))
+    Bv = dev_view(B.reshape(-1, order="F"), (K, N), (1, K))
+    Cv = dev_view(np.zeros(M * N), (M, N), (1, M))
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), path_modes=qs.PathModes("never", "never"))
+    qs.execute(plan, 1.0, 0.0)
+    torch.cuda.synchronize()
+    assert rel_fro(values(Cv), A @ B) <= 1e-13
+    del st
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
 def test_every_tile_shape(oracle, tile):
     case(f"tile override {tile}")
     rng = np.random.default_rng(20 + tile)
