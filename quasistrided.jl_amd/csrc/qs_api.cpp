@@ -583,6 +583,9 @@ qs_status qs_plan_destroy(qs_plan *p) {
   if (p->d_tables) (void)hipFree(p->d_tables);
   if (p->d_partials) (void)hipFree(p->d_partials);
   if (p->d_split) (void)hipFree(p->d_split);
+  if (p->split_side) (void)hipStreamDestroy(p->split_side);
+  if (p->split_fork) (void)hipEventDestroy(p->split_fork);
+  if (p->split_join) (void)hipEventDestroy(p->split_join);
   for (int i = 0; i < 3; ++i)
     if (p->stage[i]) (void)hipFree(p->stage[i]);
   delete p;

@@ -32,6 +32,10 @@ struct qs_plan {
   // Split precision: residues, scales and the sampled error report.
   void *d_split = nullptr;
   int64_t split_bytes = 0, split_Mp = 0, split_Np = 0, split_Kp = 0;
+  // B's norm / residue passes run on a side stream beside A's (default
+  // priority), forked from and joined back to the caller's stream by events.
+  hipStream_t split_side = nullptr;
+  hipEvent_t split_fork = nullptr, split_join = nullptr;
   int split_report_on = 0;
   struct { int32_t samples; double rel_fro, max_rel; } split_report = {0, 0.0, 0.0};
   // Host-memory staging buffers (A, B, C spans) and their capacities in bytes.

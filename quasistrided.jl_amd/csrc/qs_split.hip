@@ -204,13 +204,10 @@ __global__ __launch_bounds__(512) void crt_norm_kernel(CrtOperand o) {
   }
 }
 
-// Pass 1b: one thread per row of A, then of B, merges the row's partials in
-// chunk order into its scale: q = T / f and 2^sh with norm = f 2^(e + en0).
-__global__ __launch_bounds__(256) void crt_scale_kernel(CrtOperand oa, CrtOperand ob) {
-  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const bool isa = g < oa.R;
-  const CrtOperand &o = isa ? oa : ob;
-  const int64_t r = isa ? g : g - oa.R;
+// Pass 1b: one thread per row merges the row's partials in chunk order into
+// its scale: q = T / f and 2^sh with norm = f 2^(e + en0).
+__global__ __launch_bounds__(256) void crt_scale_kernel(CrtOperand o) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= o.R) return;
   Ssq a = Ssq{0.0, -2000, 0};
   for (int c0 = 0; c0 < o.nkc; c0 += 8) {
@@ -491,12 +488,13 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
   }
   const GemmArgs &g = a.g;
   const QS_GLOBAL unsigned int *u = (const QS_GLOBAL unsigned int *)a.u;
-  // The residue words of two blocks are loaded before any arithmetic on them.
+  // The residue words of RB blocks are loaded before any arithmetic on them.
+  constexpr int RB = 2;   // four measured slower (102 against 95 us at 4096 x 4096, 13 moduli)
 #pragma unroll 1
-  for (int b0 = 0; b0 < 4; b0 += 2) {
-  unsigned int wb[2][kMaxModuli];
+  for (int b0 = 0; b0 < 4; b0 += RB) {
+  unsigned int wb[RB][kMaxModuli];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < RB; ++h) {
     const int b = b0 + h;
     const int64_t blk = (MFAST ? mb0 + b : mb0) * a.NB + (MFAST ? nb0 : nb0 + b);
     const QS_GLOBAL unsigned int *up = u + blk * c.n * 64 + lane;
@@ -504,7 +502,7 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
     for (int l = 0; l < kMaxModuli; ++l) wb[h][l] = l < c.n ? up[l * 64] : 0u;
   }
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < RB; ++h) {
     const int b = b0 + h;
     const unsigned int (&w)[kMaxModuli] = wb[h];
     double s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
@@ -756,6 +754,10 @@ qs_status split_prepare(qs_plan *p) {
   p->split_Kp = rup(std::max<int64_t>(p->K, 1), 256);   // a multiple of four 64-K stages (the K loop's unroll)
   const SplitLayout L = split_layout(p);
   if (hipMalloc(&p->d_split, L.total) != hipSuccess) return fail(QS_ERR_MEMORY, "plan_contract: split workspace");
+  if (hipStreamCreateWithFlags(&p->split_side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&p->split_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->split_join, hipEventDisableTiming) != hipSuccess)
+    return fail(QS_ERR_DEVICE, "plan_contract: split side stream");
   p->split_bytes = (int64_t)L.total;
   p->workspace_bytes += (int64_t)L.total;
   return QS_OK;
@@ -799,12 +801,21 @@ hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t 
   ob.X = g.B; ob.type = g.b_type; ob.kfast = p->b_kfast; ob.base = g.b_base; ob.rowtab = g.nB; ob.ktab = g.kB;
   ob.R = p->N; ob.Rp = p->split_Np; ob.side = 1;
   ob.part = (Ssq *)(base + L.partB); ob.scale = (RowScale *)(base + L.scB); ob.res = (int8_t *)(base + L.rb);
-  for (const CrtOperand *o : {&oa, &ob})
-    hipLaunchKernelGGL(crt_norm_kernel, dim3((unsigned)((o->R + 63) / 64), (unsigned)nkc), dim3(512), 0, s, *o);
-  hipLaunchKernelGGL(crt_scale_kernel, dim3((unsigned)((oa.R + ob.R + 255) / 256)), dim3(256), 0, s, oa, ob);
-  for (const CrtOperand *o : {&oa, &ob})
-    hipLaunchKernelGGL(crt_residue_kernel, dim3((unsigned)(o->Rp / 64), (unsigned)(o->Kp / 64)), dim3(256), 0, s,
-                       *o, c);
+  // A's passes on the caller's stream, B's on the side stream: both are
+  // HBM-bound, and side by side their ramps and tails overlap.
+  hipError_t fe = hipEventRecord(p->split_fork, s);
+  if (fe == hipSuccess) fe = hipStreamWaitEvent(p->split_side, p->split_fork, 0);
+  if (fe != hipSuccess) return fe;
+  for (int side = 0; side < 2; ++side) {
+    const CrtOperand &o = side ? ob : oa;
+    hipStream_t os = side ? p->split_side : s;
+    hipLaunchKernelGGL(crt_norm_kernel, dim3((unsigned)((o.R + 63) / 64), (unsigned)nkc), dim3(512), 0, os, o);
+    hipLaunchKernelGGL(crt_scale_kernel, dim3((unsigned)((o.R + 255) / 256)), dim3(256), 0, os, o);
+    hipLaunchKernelGGL(crt_residue_kernel, dim3((unsigned)(o.Rp / 64), (unsigned)(o.Kp / 64)), dim3(256), 0, os, o, c);
+  }
+  fe = hipEventRecord(p->split_join, p->split_side);
+  if (fe == hipSuccess) fe = hipStreamWaitEvent(s, p->split_join, 0);
+  if (fe != hipSuccess) return fe;
 
   CrtGemmArgs ga;
   ga.ra = oa.res;
