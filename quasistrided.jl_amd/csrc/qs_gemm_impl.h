@@ -291,8 +291,10 @@ struct Gather {
     for (int r = 0; r < NK; ++r) {
       const int64_t gk = k0 + kk0 + r * KSTEP;
       if constexpr (BUF) {
+        // The element offset as loaded: scaling it here would make the
+        // compiler wait for the table load before the stage's own loads.
         const int64_t o = UNIFORM_K ? ((qs_const_i64)ktab)[gk] : ktab[gk];
-        koff[S % KS][r] = (Off)((uint32_t)o * (uint32_t)ESZ);
+        koff[S % KS][r] = (Off)(uint32_t)o;
       } else {
         kok[S % KS][r] = gk < kend;
         const int64_t gkc = gk < kend ? gk : kend - 1;
@@ -307,7 +309,7 @@ struct Gather {
     for (int r = 0; r < NL; ++r) {
       const int ir = KF ? r : 0, kr = KF ? 0 : r;
       if constexpr (BUF) {
-        const uint32_t off = rowoff[ir] + koff[S % KS][kr];
+        const uint32_t off = rowoff[ir] + koff[S % KS][kr] * (uint32_t)ESZ;
         // The loaded vector is cast whole to a scalar or a struct before any
         // part is taken: extracting its lanes directly lets the compiler
         // narrow the load to one dword (and then read the wrong lane).
@@ -502,8 +504,8 @@ struct TileGemm {
           }
         }
       };
-      // Two sets: four (loads four stages ahead) measured slower on the
-      // 32 x 64 FP64 tile, 75 against 71 us at M = N = K = 1024.
+      // Two sets.  Four (loads four stages ahead) measured slower on the
+      // 32 x 64 FP64 tile at M = N = K = 1024: 59-60 against 56 us.
       deep(std::integral_constant<int, 2>());
     } else {
       // One stage in flight: stage j + 1 is loaded while stage j multiplies.
