@@ -137,6 +137,58 @@ def bench_split(qs, torch, dev, d=64, reps=10):
     return out
 
 
+def bench_batched(qs, torch, dev, reps=20):
+    """Batch labels: C[i,j,b] := A[i,k,b] * B[k,j,b], F64, column-major, as one
+    grouped launch (and its HIP graph), against torch.bmm on the same data
+    (the vendor's strided-batched FP64 GEMM, operands laid out batch-major
+    for it).  Two shapes: 2048 batches of 64^3 and 256 of 256^3."""
+    SV = qs.StridedView
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    out = {}
+    for (n, b) in ((64, 2048), (256, 256)):
+        i = j = k = n
+        A = torch.randn(i * k * b, dtype=torch.float64, device=dev, generator=g)
+        B = torch.randn(k * j * b, dtype=torch.float64, device=dev, generator=g)
+        Cst = torch.zeros(i * j * b, dtype=torch.float64, device=dev)
+        cm = lambda sz: tuple(int(np.prod(sz[:d])) for d in range(len(sz)))
+        bp = qs.plan_batched(SV(Cst, (i, j, b), cm((i, j, b))), SV(A, (i, k, b), cm((i, k, b))), (1, 2, 3),
+                             SV(B, (k, j, b), cm((k, j, b))), (2, 4, 3), (1, 4, 3))
+        flops = 2 * i * j * k * b
+        tune = bp.autotune(reps=5)
+        graph = bp.capture(repeats=1, alpha=1.0, beta=0.0)
+        graph.launch()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            graph.launch()
+        e1.record()
+        torch.cuda.synchronize()
+        t_graph = e0.elapsed_time(e1) * 1e-3 / reps
+        # torch.bmm over [b, i, k] x [b, k, j] (the column-major slices, transposed views made contiguous)
+        At = A.view(b, k, i).transpose(1, 2).contiguous()
+        Bt = B.view(b, j, k).transpose(1, 2).contiguous()
+        torch.bmm(At, Bt)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            Ct = torch.bmm(At, Bt)
+        e1.record()
+        torch.cuda.synchronize()
+        t_bmm = e0.elapsed_time(e1) * 1e-3 / reps
+        err = (Ct.transpose(1, 2).reshape(-1) - Cst).norm().item() / Ct.norm().item()
+        out[f"{b}x{n}^3"] = {"graph_us": round(t_graph * 1e6, 1), "graph_tflops": round(flops / t_graph / 1e12, 2),
+                             "graph_mfma_frac": round(flops / t_graph / PEAK_FP64, 4),
+                             "torch_bmm_us": round(t_bmm * 1e6, 1), "torch_bmm_tflops": round(flops / t_bmm / 1e12, 2),
+                             "vs_torch_bmm": round(t_bmm / t_graph, 3), "rel_fro_vs_bmm": err,
+                             "tile": bp.tile, "tile_us": {str(t): round(v * 1e6, 1) for t, v in tune.items()}}
+        graph.release()
+        bp.release()
+        del A, B, Cst, At, Bt, Ct
+    return out
+
+
 def bench_bandwidth(qs, torch, dev, reps=20):
     """The HBM-bound paths at sizes well past the 4 MB L2 of an XCD (and the
     MALL for the large operand), F64, operands N(0,1): algorithmic bytes per
@@ -462,7 +514,7 @@ def main():
     ap.add_argument("--tune-cache", default=None,
                     help="autotune cache file: loaded if it exists (no tuning runs), else written after tuning")
     ap.add_argument("--only", default=None,
-                    choices=["split", "grouped_f64", "grouped_c64", "served", "bandwidth"],
+                    choices=["split", "grouped_f64", "grouped_c64", "served", "bandwidth", "batched"],
                     help="only run one of the extra workloads and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy 5 times (PMC byte-counter calibration)")
@@ -493,6 +545,8 @@ def main():
             out = bench_split(qs, torch, dev)
         elif args.only == "bandwidth":
             out = bench_bandwidth(qs, torch, dev)
+        elif args.only == "batched":
+            out = bench_batched(qs, torch, dev)
         elif args.only.startswith("grouped"):
             out = bench_grouped(qs, torch, dev, args.only.endswith("c64"))
         else:
@@ -587,6 +641,7 @@ def main():
             del p, vw
         extra["split_rank4_all64_f64"] = bench_split(qs, torch, dev)
         extra["bandwidth_f64"] = bench_bandwidth(qs, torch, dev)
+        extra["batched_f64"] = bench_batched(qs, torch, dev)
         extra["grouped_512_mps_f64"] = bench_grouped(qs, torch, dev, False)
         extra["grouped_512_mps_c64"] = bench_grouped(qs, torch, dev, True)
         pools = served_pools(qs, torch, dev)

@@ -270,6 +270,15 @@ typedef struct qs_graph qs_graph;
  * heuristic), or NULL for all heuristic. */
 qs_status qs_group_create(qs_plan *const *plans, int32_t n, const int32_t *tiles, qs_group_plan **out);
 
+/* A batched contraction as a group: n members that all run `plan`, member i
+ * with a_offsets[i] / b_offsets[i] / c_offsets[i] (elements) added to the
+ * plan's A / B / C bases.  This is the device form of batch labels (a label
+ * in A, B and C), which the reference rejects (src/planning/labels.jl:48-54,
+ * the check `plan_contract` keeps); it is an extension with no counterpart
+ * there.  Members' C spans must not overlap.  `tile` as in qs_group_create. */
+qs_status qs_group_create_batched(qs_plan *plan, int32_t n, const int64_t *a_offsets, const int64_t *b_offsets,
+                                  const int64_t *c_offsets, int32_t tile, qs_group_plan **out);
+
 /* Run every member: C_i = alpha_i A_i B_i + beta_i C_i.  alphas / betas hold
  * two doubles per member (re, im; im ignored for real members); NULL means
  * alpha = 1 / beta = 0.  Spans of every member are checked before launch. */

@@ -17,6 +17,7 @@ from .backend import QuasiStridedBackend, contraction_labels, tensorcontract
 from .add import tensoradd, tensortrace
 from .target import target_profile
 from .group import GroupPlan, Graph
+from .batch import BatchedPlan, batch_labels, contract_batched, plan_batched
 from .server import Server
 from . import _lib
 
@@ -27,5 +28,6 @@ __all__ = [
     "PathModes", "autotune", "autotune_clear", "autotune_load", "autotune_save", "compute_type", "contract", "contract_", "error_report", "set_error_report", "execute", "execute_", "isconj", "op_conjugates",
     "plan_contract", "planned", "release", "QuasiStridedBackend", "contraction_labels", "tensorcontract",
     "tensoradd", "tensortrace",
-    "target_profile", "GroupPlan", "Graph", "Server",
+    "target_profile", "GroupPlan", "Graph", "Server", "BatchedPlan", "batch_labels", "contract_batched",
+    "plan_batched",
 ]

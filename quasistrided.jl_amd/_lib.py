@@ -120,6 +120,8 @@ SIGNATURES = {
     "qs_autotune_load": (C.c_int, [C.c_char_p]),
     "qs_autotune_clear": (C.c_int, []),
     "qs_group_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_void_p)]),
+    "qs_group_create_batched": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_void_p)]),
     "qs_group_execute": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]),
     "qs_group_flops": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "qs_group_destroy": (C.c_int, [C.c_void_p]),

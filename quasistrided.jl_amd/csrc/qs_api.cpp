@@ -456,6 +456,8 @@ GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *
   g.a_bytes = buf ? (uint32_t)a_bytes : 0;
   g.b_bytes = buf ? (uint32_t)b_bytes : 0;
   g.mixed = buf ? 0 : 1;
+  g.batch_off = nullptr;
+  g.batch = 1;
   return g;
 }
 

@@ -23,6 +23,7 @@
 //     never reads C.
 #pragma once
 
+#include <algorithm>
 #include <type_traits>
 
 #include "qs_internal.h"
@@ -711,9 +712,51 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmArgs g) {
   if (g.k_splits > 1) splitk_fixup<R, CPLX, MIXED, BM, BN>(g, bid, tm, tn, &last);
 }
 
+// A batched launch (single storage type, no split-K): workgroup (tile,
+// blockIdx.z) runs the tile of batch blockIdx.z, whose offsets move the three
+// bases.
+template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF>
+__global__ __launch_bounds__(256, 2) void gemm_batched_kernel(const GemmArgs g) {
+  typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, false> TG;
+  __shared__ R smem[TG::SMEM];
+  const int64_t tiles_m = (g.M + BM - 1) / BM;
+  const int64_t tiles_n = (g.N + BN - 1) / BN;
+  const int64_t bid = xcd_swizzle(blockIdx.x, (int64_t)gridDim.x);
+  const int64_t tm = bid % tiles_m, tn = bid / tiles_m;
+  if (tn >= tiles_n) return;
+  GemmArgs gz = g;
+  const int64_t *o = g.batch_off + 3 * (int64_t)blockIdx.z;
+  gz.a_base += o[0];
+  gz.a_lo += o[0];
+  gz.b_base += o[1];
+  gz.b_lo += o[1];
+  gz.c_base += o[2];
+  TG::template run<true>(gz, tm, tn, 0, smem);
+}
+
 template <typename R, bool CPLX, int BM, int BN, bool MIXED>
 static hipError_t launch_tile_mixed(const GemmArgs &g, hipStream_t s) {
   const int64_t tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
+  if constexpr (!MIXED) {
+    if (g.batch_off) {   // batched: at most 65535 batches per launch
+      for (int64_t z0 = 0; z0 < g.batch; z0 += 65535) {
+        GemmArgs part = g;
+        part.batch_off = g.batch_off + 3 * z0;
+        const dim3 bgrid((unsigned)tiles, 1, (unsigned)std::min<int64_t>(65535, g.batch - z0));
+        if (g.a_kfast && g.b_kfast)
+          hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, true>), bgrid, dim3(256), 0, s, part);
+        else if (g.a_kfast)
+          hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, false>), bgrid, dim3(256), 0, s, part);
+        else if (g.b_kfast)
+          hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, false, true>), bgrid, dim3(256), 0, s, part);
+        else
+          hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, false, false>), bgrid, dim3(256), 0, s, part);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+      }
+      return hipSuccess;
+    }
+  }
   dim3 grid((unsigned)tiles, (unsigned)g.k_splits), block(256);
   if (g.a_kfast && g.b_kfast)
     hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, true, true, MIXED>), grid, block, 0, s, g);

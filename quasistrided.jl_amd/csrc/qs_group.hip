@@ -133,9 +133,15 @@ struct Launch {   // one kernel instantiation's share of the group
 struct qs_group_plan {
   std::vector<qs_plan *> plans;
   std::vector<int> tile_override;       // per plan, -1 = heuristic
+  // Batched groups: element offsets added to each member's A / B / C bases
+  // (empty for a group of separate plans).  With equal coefficients for all
+  // members they run as one batched launch of the plan's own kernel
+  // (batch_args, offsets in d_batch_off); otherwise as a grouped launch.
+  std::vector<int64_t> a_off, b_off, c_off;
+  int64_t *d_batch_off = nullptr;
+  qs::GemmArgs batch_args;
   std::vector<qs::Launch> launches;
-  std::vector<qs_plan *> standalone;    // members with mixed storage types
-  std::vector<int> standalone_index;
+  std::vector<int> standalone;          // members on the generic gather, run one by one
   double *d_coef = nullptr;             // 4 doubles per plan
   double *h_coef = nullptr;             // pinned host staging
   std::vector<double> last_coef;
@@ -167,13 +173,24 @@ static void free_launches(qs_group_plan *g) {
   g->standalone.clear();
 }
 
+// Member i's storage bases.
+static int64_t base_a(const qs_group_plan *g, size_t i) {
+  return g->plans[i]->desc.a_base + (g->a_off.empty() ? 0 : g->a_off[i]);
+}
+static int64_t base_b(const qs_group_plan *g, size_t i) {
+  return g->plans[i]->desc.b_base + (g->b_off.empty() ? 0 : g->b_off[i]);
+}
+static int64_t base_c(const qs_group_plan *g, size_t i) {
+  return g->plans[i]->desc.c_base + (g->c_off.empty() ? 0 : g->c_off[i]);
+}
+
 // (Re)build the device tables from the plans and their tile classes.
 static qs_status build_tables(qs_group_plan *g) {
   free_launches(g);
   std::map<std::tuple<int, int, int, int>, std::vector<GroupItem>> classes;
   for (size_t i = 0; i < g->plans.size(); ++i) {
     qs_plan *p = g->plans[i];
-    GemmArgs a = make_args(p, p->desc.a, p->desc.a_base, p->desc.b, p->desc.b_base, p->desc.c, p->desc.c_base);
+    GemmArgs a = make_args(p, p->desc.a, base_a(g, i), p->desc.b, base_b(g, i), p->desc.c, base_c(g, i));
     a.k_splits = 1;
     a.k_chunk = p->K;
     a.partials = nullptr;
@@ -181,8 +198,8 @@ static qs_status build_tables(qs_group_plan *g) {
     GroupItem it;
     std::memset(&it, 0, sizeof it);
     it.g = a;
-    if (a.mixed) {   // the generic gather (mixed types, spans past 4 GB) runs standalone, on the same stream
-      g->standalone.push_back(p);
+    if (a.mixed) {   // the generic gather (mixed types, spans past 2 GiB) runs standalone, on the same stream
+      g->standalone.push_back((int)i);
       continue;
     }
     it.cls = g->tile_override[i] >= 0 ? g->tile_override[i] : grouped_tile(p->M, p->N, p->desc.compute_type);
@@ -247,17 +264,34 @@ static qs_status set_coefficients(qs_group_plan *g, const double *alphas, const 
 }
 
 static qs_status launch_all(qs_group_plan *g, hipStream_t s) {
+  if (g->d_batch_off && !g->plans.empty() && !g->batch_args.mixed) {
+    // One batched launch when every member has the same coefficients (and
+    // the plan takes the buffer-load gather).
+    const std::vector<double> &c = g->last_coef;
+    bool same = true;
+    for (size_t i = 4; i < c.size() && same; ++i) same = c[i] == c[i % 4];
+    if (same) {
+      GemmArgs a = g->batch_args;
+      a.alpha_re = c[0];
+      a.alpha_im = c[1];
+      a.beta_re = c[2];
+      a.beta_im = c[3];
+      a.beta_case = (c[2] == 0 && c[3] == 0) ? 0 : (c[2] == 1 && c[3] == 0) ? 1 : 2;
+      if (c[0] == 0 && c[1] == 0) a.K = 0;   // alpha == 0: C = beta C, A and B never read
+      hipError_t e = launch_gemm(a, s);
+      if (e != hipSuccess) return fail(QS_ERR_DEVICE, std::string("group: batched launch: ") + hipGetErrorString(e));
+      return QS_OK;
+    }
+  }
   for (const Launch &l : g->launches) {
     if (l.total == 0) continue;
     hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, l.d_items, l.n, l.total, g->d_coef, l.grid, s);
     if (e != hipSuccess) return fail(QS_ERR_DEVICE, std::string("group: launch: ") + hipGetErrorString(e));
   }
-  for (qs_plan *p : g->standalone) {
-    size_t i = 0;
-    while (g->plans[i] != p) ++i;
+  for (int i : g->standalone) {
+    qs_plan *p = g->plans[i];
     const double *c = g->last_coef.data() + 4 * i;
-    qs_status st = run_device(p, p->desc.a, p->desc.a_base, p->desc.b, p->desc.b_base, p->desc.c, p->desc.c_base,
-                              c, c + 2, s);
+    qs_status st = run_device(p, p->desc.a, base_a(g, i), p->desc.b, base_b(g, i), p->desc.c, base_c(g, i), c, c + 2, s);
     if (st != QS_OK) return st;
   }
   return QS_OK;
@@ -332,20 +366,25 @@ using namespace qs;
 
 extern "C" {
 
+static qs_status check_member(const qs_plan *p, int32_t i, int32_t tile) {
+  if (!p) return fail(QS_ERR_ARGUMENT, "qs_group_create: null plan " + std::to_string(i));
+  if (p->desc.memory != QS_MEM_DEVICE)
+    return fail(QS_ERR_ARGUMENT, "qs_group_create: plan " + std::to_string(i) + " is not in device memory");
+  if (p->path != QS_PATH_GEMM && p->path != QS_PATH_SKINNY)
+    return fail(QS_ERR_ARGUMENT, "qs_group_create: plan " + std::to_string(i) +
+                                     " does not run the tiled GEMM path (degenerate extents)");
+  if (tile < -1 || tile >= TILE_COUNT)
+    return fail(QS_ERR_ARGUMENT, "qs_group_create: bad tile for plan " + std::to_string(i));
+  return QS_OK;
+}
+
 qs_status qs_group_create(qs_plan *const *plans, int32_t n, const int32_t *tiles, qs_group_plan **out) {
   if (!out) return fail(QS_ERR_ARGUMENT, "qs_group_create: null output");
   *out = nullptr;
   if (n < 0 || (n > 0 && !plans)) return fail(QS_ERR_ARGUMENT, "qs_group_create: bad plan array");
   for (int32_t i = 0; i < n; ++i) {
-    const qs_plan *p = plans[i];
-    if (!p) return fail(QS_ERR_ARGUMENT, "qs_group_create: null plan " + std::to_string(i));
-    if (p->desc.memory != QS_MEM_DEVICE)
-      return fail(QS_ERR_ARGUMENT, "qs_group_create: plan " + std::to_string(i) + " is not in device memory");
-    if (p->path != QS_PATH_GEMM && p->path != QS_PATH_SKINNY)
-      return fail(QS_ERR_ARGUMENT, "qs_group_create: plan " + std::to_string(i) +
-                                       " does not run the tiled GEMM path (degenerate extents)");
-    if (tiles && (tiles[i] < -1 || tiles[i] >= TILE_COUNT))
-      return fail(QS_ERR_ARGUMENT, "qs_group_create: bad tile for plan " + std::to_string(i));
+    const qs_status st = check_member(plans[i], i, tiles ? tiles[i] : -1);
+    if (st != QS_OK) return st;
   }
   qs_group_plan *g = new qs_group_plan();
   g->plans.assign(plans, plans + n);
@@ -366,15 +405,69 @@ qs_status qs_group_create(qs_plan *const *plans, int32_t n, const int32_t *tiles
   return QS_OK;
 }
 
+qs_status qs_group_create_batched(qs_plan *plan, int32_t n, const int64_t *a_offsets, const int64_t *b_offsets,
+                                  const int64_t *c_offsets, int32_t tile, qs_group_plan **out) {
+  if (!out) return fail(QS_ERR_ARGUMENT, "qs_group_create_batched: null output");
+  *out = nullptr;
+  if (!plan || n < 0 || (n > 0 && (!a_offsets || !b_offsets || !c_offsets)))
+    return fail(QS_ERR_ARGUMENT, "qs_group_create_batched: null plan or offset array");
+  qs_status st = check_member(plan, 0, tile);
+  if (st != QS_OK) return st;
+  qs_group_plan *g = new qs_group_plan();
+  g->plans.assign((size_t)n, plan);
+  g->tile_override.assign((size_t)n, tile);
+  g->a_off.assign(a_offsets, a_offsets + n);
+  g->b_off.assign(b_offsets, b_offsets + n);
+  g->c_off.assign(c_offsets, c_offsets + n);
+  g->flops = (int64_t)n * (is_complex(plan->desc.compute_type) ? 8 : 2) * plan->M * plan->N * plan->K;
+  const size_t cb = sizeof(double) * 4 * (size_t)std::max<int32_t>(n, 1);
+  if (hipMalloc(&g->d_coef, cb) != hipSuccess || hipHostMalloc(&g->h_coef, cb, hipHostMallocDefault) != hipSuccess) {
+    qs_group_destroy(g);
+    return fail(QS_ERR_MEMORY, "qs_group_create_batched: coefficient buffers");
+  }
+  st = build_tables(g);
+  if (st != QS_OK) {
+    qs_group_destroy(g);
+    return st;
+  }
+  if (n > 0) {
+    std::vector<int64_t> off(3 * (size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+      off[3 * i] = a_offsets[i];
+      off[3 * i + 1] = b_offsets[i];
+      off[3 * i + 2] = c_offsets[i];
+    }
+    if (hipMalloc(&g->d_batch_off, sizeof(int64_t) * off.size()) != hipSuccess ||
+        hipMemcpy(g->d_batch_off, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      qs_group_destroy(g);
+      return fail(QS_ERR_MEMORY, "qs_group_create_batched: offset table");
+    }
+    GemmArgs a = make_args(plan, plan->desc.a, plan->desc.a_base, plan->desc.b, plan->desc.b_base, plan->desc.c,
+                           plan->desc.c_base);
+    a.k_splits = 1;
+    a.k_chunk = plan->K;
+    a.partials = nullptr;
+    a.tile_counters = nullptr;
+    if (tile >= 0) a.tile = tile;
+    a.batch_off = g->d_batch_off;
+    a.batch = n;
+    g->batch_args = a;
+  }
+  *out = g;
+  return QS_OK;
+}
+
 qs_status qs_group_execute(qs_group_plan *g, const double *alphas, const double *betas, void *stream) {
   if (!g) return fail(QS_ERR_ARGUMENT, "qs_group_execute: null group");
   hipStream_t s = (hipStream_t)stream;
   // Exact span checks for every member before anything is launched.
-  for (qs_plan *p : g->plans) {
+  for (size_t i = 0; i < g->plans.size(); ++i) {
+    const qs_plan *p = g->plans[i];
     const qs_contract_desc &d = p->desc;
-    const int64_t clo = d.c_base + p->mC_lo + p->nC_lo, chi = d.c_base + p->mC_hi + p->nC_hi;
-    const int64_t alo = d.a_base + p->mA_lo + p->kA_lo, ahi = d.a_base + p->mA_hi + p->kA_hi;
-    const int64_t blo = d.b_base + p->kB_lo + p->nB_lo, bhi = d.b_base + p->kB_hi + p->nB_hi;
+    const int64_t cb = base_c(g, i), ab = base_a(g, i), bb = base_b(g, i);
+    const int64_t clo = cb + p->mC_lo + p->nC_lo, chi = cb + p->mC_hi + p->nC_hi;
+    const int64_t alo = ab + p->mA_lo + p->kA_lo, ahi = ab + p->mA_hi + p->kA_hi;
+    const int64_t blo = bb + p->kB_lo + p->nB_lo, bhi = bb + p->kB_hi + p->nB_hi;
     if (clo < 0 || chi >= d.c_len || alo < 0 || ahi >= d.a_len || blo < 0 || bhi >= d.b_len)
       return fail(QS_ERR_BOUNDS, "qs_group_execute: a member's operand span exceeds its storage");
   }
@@ -394,6 +487,7 @@ qs_status qs_group_destroy(qs_group_plan *g) {
   free_launches(g);
   if (g->d_coef) (void)hipFree(g->d_coef);
   if (g->h_coef) (void)hipHostFree(g->h_coef);
+  if (g->d_batch_off) (void)hipFree(g->d_batch_off);
   delete g;
   return QS_OK;
 }

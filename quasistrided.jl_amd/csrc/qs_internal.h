@@ -55,6 +55,11 @@ struct GemmArgs {
   int *tile_counters;     // k_splits > 1: per-tile arrivals, zero between launches
   int64_t a_lo, b_lo;     // accessed spans of A and B (elements from the storage start) ...
   uint32_t a_bytes, b_bytes;   // ... and their lengths in bytes, for the buffer-load gather
+  // A batched launch (k_splits == 1, mixed == 0): batch contractions, batch
+  // b with batch_off[3b .. 3b+2] elements added to the A, B and C bases.
+  // batch_off null: one contraction.
+  const int64_t *batch_off;
+  int64_t batch;
 };
 
 // Largest operand span, in bytes, that the buffer-load gather addresses: a
