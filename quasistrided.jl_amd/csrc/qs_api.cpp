@@ -348,12 +348,10 @@ qs_status configure_gemm(qs_plan *p, int tile, int splits) {
     chunk = cdiv(cdiv(p->K, splits), 2 * ts.bk) * 2 * ts.bk;
     ks = (int)cdiv(p->K, chunk);
   }
-  // Partial sums, then one arrival counter per tile (256-byte aligned).
-  const size_t part = ks > 1 ? (size_t)ks * (is_complex(ct) ? 2 : 1) * (size_t)p->M * (size_t)p->N *
+  // Partial sums of every split.
+  const size_t need = ks > 1 ? (size_t)ks * (is_complex(ct) ? 2 : 1) * (size_t)p->M * (size_t)p->N *
                                    bytes_of_compute_real(ct)
                              : 0;
-  const size_t cnt_off = (part + 255) / 256 * 256;
-  const size_t need = ks > 1 ? cnt_off + sizeof(int) * (size_t)tiles : 0;
   if (need > (size_t)p->workspace_bytes) {
     if (p->d_partials) (void)hipFree(p->d_partials);
     p->d_partials = nullptr;
@@ -367,15 +365,6 @@ qs_status configure_gemm(qs_plan *p, int tile, int splits) {
       p->workspace_bytes = (int64_t)need;
     }
   }
-  if (ks > 1) {
-    // The counters start at zero; every launch's last arrival per tile
-    // resets its own.  (Set up here, never inside a caller's stream.)
-    if (hipDeviceSynchronize() != hipSuccess ||
-        hipMemset((char *)p->d_partials + cnt_off, 0, sizeof(int) * (size_t)tiles) != hipSuccess ||
-        hipDeviceSynchronize() != hipSuccess)
-      return fail(QS_ERR_DEVICE, "plan_contract: split-K counters");
-  }
-  p->counter_off = (int64_t)cnt_off;
   p->tile = tile;
   p->k_splits = ks;
   p->k_chunk = chunk;
@@ -442,7 +431,6 @@ GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *
   g.k_splits = p->k_splits;
   g.k_chunk = p->k_chunk;
   g.partials = p->d_partials;
-  g.tile_counters = p->k_splits > 1 ? (int *)((char *)p->d_partials + p->counter_off) : nullptr;
   g.tile = p->tile;
   g.a_kfast = p->a_kfast;
   g.b_kfast = p->b_kfast;

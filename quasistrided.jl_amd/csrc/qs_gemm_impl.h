@@ -8,8 +8,8 @@
 //   * one workgroup (4 waves, 2x2) per BM x BN tile of C, full K (or one K
 //     split) accumulated in MFMA registers -- beta applied once, in the
 //     epilogue, by construction (nest.jl:83 applies it on the first K block);
-//     with split-K the last split of a tile to finish adds the partials in a
-//     fixed order and runs the epilogue;
+//     with split-K every split stores its partial sums and a second kernel
+//     adds them in a fixed order and runs the epilogue;
 //   * the pack is a gather fused into the global->LDS load: address =
 //     base + rowoffset[i] + koffset[k] (Tile, src/layout/tiles.jl:102-103),
 //     lanes laid along whichever of the free or the K group is unit-stride in
@@ -638,24 +638,22 @@ struct TileGemm {
   }
 };
 
-// Split-K: every split of a tile stores its partial sums, then counts itself
-// in on the tile's counter (release); the last to arrive (acquire) adds the
-// partials in split order 0, 1, ..., S-1 -- a fixed order, so results do not
-// depend on scheduling -- applies the epilogue and resets the counter.  No
-// workgroup waits for another.
+// Split-K: every split of a tile stores its partial sums; this kernel, one
+// workgroup per tile launched after the GEMM on the same stream, adds them in
+// split order 0, 1, ..., S-1 (a fixed order, so results do not depend on
+// scheduling) and applies the epilogue.  The kernel boundary orders the
+// partial stores before these loads.  (A last-arriver reduction inside the
+// GEMM needs an agent-scope release and acquire per workgroup, which on a
+// part with one L2 per XCD writes back and invalidates L2 each time.)
 template <typename R, bool CPLX, bool MIXED, int BM, int BN>
-__device__ __forceinline__ void splitk_fixup(const GemmArgs &g, int64_t tile, int64_t tm, int64_t tn, int *last) {
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const GemmArgs g) {
   constexpr int P = CPLX ? 2 : 1;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) *last = atomicAdd(&g.tile_counters[tile], 1) == g.k_splits - 1;
-  __syncthreads();
-  if (!*last) return;
-  __threadfence();
+  const int64_t tiles_m = (g.M + BM - 1) / BM;
+  const int64_t tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
   const int64_t plane = g.M * g.N;
   const QS_GLOBAL R *part = (const QS_GLOBAL R *)g.partials;
-  // Batches of 8 elements per thread: every partial and old-C load of a
-  // batch is issued before its sums and stores.
+  // Batches of elements per thread: every partial and old-C load of a batch
+  // is issued before its sums and stores.
   constexpr int EPT = BM * BN / 256, CW = (CPLX || MIXED || BM * BN >= 128 * 128) ? 2 : 8, CH = EPT < CW ? EPT : CW;
 #pragma unroll 1
   for (int c0 = 0; c0 < EPT; c0 += CH) {
@@ -695,21 +693,18 @@ __device__ __forceinline__ void splitk_fixup(const GemmArgs &g, int64_t tile, in
     for (int j = 0; j < CH; ++j)
       if (ok[j]) store_c_at<R, CPLX, MIXED>(g, cidx[j], xr[j], xi[j], o[j], oi[j]);
   }
-  if (threadIdx.x == 0) atomicExch(&g.tile_counters[tile], 0);
 }
 
 template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED>
 __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmArgs g) {
   typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, MIXED> TG;
   __shared__ R smem[TG::SMEM];
-  __shared__ int last;
   const int64_t tiles_m = (g.M + BM - 1) / BM;
   const int64_t tiles_n = (g.N + BN - 1) / BN;
   const int64_t bid = xcd_swizzle(blockIdx.x, (int64_t)gridDim.x);
   const int64_t tm = bid % tiles_m, tn = bid / tiles_m;
   if (tn >= tiles_n) return;
   TG::template run<true>(g, tm, tn, blockIdx.y, smem);
-  if (g.k_splits > 1) splitk_fixup<R, CPLX, MIXED, BM, BN>(g, bid, tm, tn, &last);
 }
 
 // A batched launch (single storage type, no split-K): workgroup (tile,
@@ -766,6 +761,11 @@ static hipError_t launch_tile_mixed(const GemmArgs &g, hipStream_t s) {
     hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, false, true, MIXED>), grid, block, 0, s, g);
   else
     hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, false, false, MIXED>), grid, block, 0, s, g);
+  if (g.k_splits > 1) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((splitk_reduce_kernel<R, CPLX, MIXED, BM, BN>), dim3((unsigned)tiles), block, 0, s, g);
+  }
   return hipGetLastError();
 }
 

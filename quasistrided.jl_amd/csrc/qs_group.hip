@@ -194,7 +194,6 @@ static qs_status build_tables(qs_group_plan *g) {
     a.k_splits = 1;
     a.k_chunk = p->K;
     a.partials = nullptr;
-    a.tile_counters = nullptr;
     GroupItem it;
     std::memset(&it, 0, sizeof it);
     it.g = a;
@@ -447,7 +446,6 @@ qs_status qs_group_create_batched(qs_plan *plan, int32_t n, const int64_t *a_off
     a.k_splits = 1;
     a.k_chunk = plan->K;
     a.partials = nullptr;
-    a.tile_counters = nullptr;
     if (tile >= 0) a.tile = tile;
     a.batch_off = g->d_batch_off;
     a.batch = n;

@@ -52,7 +52,6 @@ struct GemmArgs {
   int a_kfast, b_kfast;   // gather lanes along K (1) or along M / N (0)
   int mixed;              // the generic gather: some storage type differs from the compute
                           // type, or an operand span exceeds the 32-bit buffer range
-  int *tile_counters;     // k_splits > 1: per-tile arrivals, zero between launches
   int64_t a_lo, b_lo;     // accessed spans of A and B (elements from the storage start) ...
   uint32_t a_bytes, b_bytes;   // ... and their lengths in bytes, for the buffer-load gather
   // A batched launch (k_splits == 1, mixed == 0): batch contractions, batch
@@ -69,8 +68,8 @@ constexpr int64_t kBufferSpanMax = 0x7FFFFF00LL;
 // stages of at most 32 K beyond the last one it multiplies.
 constexpr int64_t kKTablePad = 192;
 
-// Launch the tiled MFMA GEMM (with k_splits > 1, the last split of each tile
-// reduces the partials in a fixed order).  Shapes and tables must already be validated.
+// Launch the tiled MFMA GEMM (with k_splits > 1, a second kernel reduces the
+// partials in a fixed order).  Shapes and tables must already be validated.
 hipError_t launch_gemm(const GemmArgs &g, hipStream_t stream);
 
 // C = beta * C over the M x N rectangle (alpha == 0 or K == 0).

@@ -23,7 +23,6 @@ struct qs_plan {
   int64_t *d_tables = nullptr;
   const int64_t *t_mA = nullptr, *t_mC = nullptr, *t_nB = nullptr, *t_nC = nullptr, *t_kA = nullptr, *t_kB = nullptr;
   void *d_partials = nullptr;
-  int64_t counter_off = 0;    // byte offset of the split-K tile counters in d_partials
   // Map ranges (exact min / max offsets of each map).
   int64_t mA_lo = 0, mA_hi = 0, mC_lo = 0, mC_hi = 0, nB_lo = 0, nB_hi = 0, nC_lo = 0, nC_hi = 0;
   int64_t kA_lo = 0, kA_hi = 0, kB_lo = 0, kB_hi = 0;

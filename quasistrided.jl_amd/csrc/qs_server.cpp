@@ -230,7 +230,6 @@ qs_status qs_server_submit(qs_server *s, qs_plan *p, const void *a, int64_t a_ba
   r.g.k_splits = 1;
   r.g.k_chunk = p->K;
   r.g.partials = nullptr;
-  r.g.tile_counters = nullptr;
   const int ct = p->desc.compute_type;
   r.cls = grouped_tile(p->M, p->N, ct);
   r.al[0] = alpha ? alpha[0] : 1.0;
