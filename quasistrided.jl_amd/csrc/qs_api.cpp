@@ -131,16 +131,7 @@ int unit_dim(const qs_group &g, int p) {
 
 // Mirrors pick_bk_for in qs_gemm_impl.h.
 static int pick_bk_host(int bm, int bn, int ct, bool akf, bool bkf) {
-  const int planes = is_complex(ct) ? 2 : 1;
-  const int rsize = is_double(ct) ? 8 : 4;
-  auto rowmajor = [&](int bk) { return rsize == 8 ? bk + 2 : ((bk / 4) % 2 ? bk : bk + 4); };
-  int best = 4;
-  for (int bk = 4; bk <= 32; bk *= 2) {
-    const int a = akf ? bm * rowmajor(bk) : bk * (bm + 16);
-    const int b = bkf ? bn * rowmajor(bk) : bk * (bn + 16);
-    if (2 * planes * (a + b) * rsize <= 76 * 1024) best = bk;
-  }
-  return best;
+  return pick_bk_for(is_double(ct) ? 8 : 4, is_complex(ct) ? 2 : 1, bm, bn, akf, bkf);
 }
 
 TileShape tile_shape(int tile, int ct, bool akf, bool bkf) {

@@ -61,6 +61,22 @@ struct GemmArgs {
   int64_t batch;
 };
 
+// Row pitch, in elements, of an LDS tile stored [row][k]: the smallest pitch
+// >= bk that keeps a fragment read free of bank conflicts.
+constexpr int lds_pitch_rowmajor(int rsize, int bk) { return rsize == 8 ? bk + 1 : ((bk / 4) % 2 ? bk : bk + 4); }
+// K depth of one LDS stage: the largest power of two in [4, 32] whose double-
+// buffered A and B tiles fit the LDS budget (76 KiB: two workgroups per CU).
+// Shared by the kernels and the host planner, which rounds split-K chunks to it.
+constexpr int pick_bk_for(int rsize, int planes, int bm, int bn, bool akf, bool bkf) {
+  int best = 4;
+  for (int bk = 4; bk <= 32; bk *= 2) {
+    const int a = akf ? bm * lds_pitch_rowmajor(rsize, bk) : bk * (bm + 16);
+    const int b = bkf ? bn * lds_pitch_rowmajor(rsize, bk) : bk * (bn + 16);
+    if (2 * planes * (a + b) * rsize <= 76 * 1024) best = bk;
+  }
+  return best;
+}
+
 // Largest operand span, in bytes, that the buffer-load gather addresses: a
 // K-table pad entry moves an offset 2^31 bytes on, past any such span.
 constexpr int64_t kBufferSpanMax = 0x7FFFFF00LL;
