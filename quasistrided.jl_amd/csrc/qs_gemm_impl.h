@@ -342,21 +342,23 @@ struct Gather {
 };
 
 // Scheduling of one K stage's basic block: its NMF MFMAs with the stage's
-// other work spread between them -- after three LDS fragment reads, each MFMA
-// is followed by one global load, two fragment reads and one LDS write of the
+// other work spread between them -- after FIRST LDS fragment reads, each MFMA
+// is followed by one global load, PER fragment reads and one LDS write of the
 // next stage.  Issued as a burst, the gather loads stalled the wave in order
 // at the memory unit and held up the MFMAs behind them: on the 1024^3 FP64
 // shape with 32 x 64 tiles, 61 -> 51 us; without any in-loop loads the same
-// kernel took 44 us.
-template <int NMF>
+// kernel took 44 us.  Waves with four or more MFMA blocks read six fragments
+// ahead and one per MFMA (64 x 64 tiles 55 -> 52 us, 64 x 128 at 4096^3
+// 2.10 -> 2.05 ms); smaller ones three ahead and two per MFMA.
+template <int NMF, int FIRST, int PER>
 __device__ __forceinline__ void sched_interleave() {
-  __builtin_amdgcn_sched_group_barrier(0x100, 3, 0);   // LDS reads
+  __builtin_amdgcn_sched_group_barrier(0x100, FIRST, 0);   // LDS reads
 #pragma unroll
   for (int i = 0; i < NMF; ++i) {
-    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
-    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   // global load
-    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);   // LDS reads
-    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);   // LDS write
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);     // MFMA
+    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);     // global load
+    __builtin_amdgcn_sched_group_barrier(0x100, PER, 0);   // LDS reads
+    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);     // LDS write
   }
 }
 
@@ -463,7 +465,7 @@ struct TileGemm {
           gb.template stage<NX, LB::plane>(Ws + SA, t, sb);
           // The stage's loads, fragment reads and LDS writes spread between
           // its MFMAs (the loads are not pinned ahead of them).
-          sched_interleave<(BK / 4) * FM * FN * (CPLX ? 4 : 1)>();
+          sched_interleave<(BK / 4) * FM * FN * (CPLX ? 4 : 1), FM * FN >= 4 ? 6 : 3, FM * FN >= 4 ? 1 : 2>();
           __syncthreads();
         };
         if (kbeg < kend) {
@@ -574,7 +576,7 @@ struct TileGemm {
         R *Ws = smem + (buf ^ 1) * (SA + SB);
         ga.template stage<0, LA::plane>(Ws, t, sa);
         gb.template stage<0, LB::plane>(Ws + SA, t, sb);
-        sched_interleave<(BK / 4) * FM * FN * (CPLX ? 4 : 1)>();
+        sched_interleave<(BK / 4) * FM * FN * (CPLX ? 4 : 1), FM * FN >= 4 ? 6 : 3, FM * FN >= 4 ? 1 : 2>();
         __syncthreads();
         buf ^= 1;
       }
