@@ -120,6 +120,9 @@ typedef struct qs_plan_info {
   int32_t moduli;        /* split precision: CRT moduli (INT8 GEMMs)      */
   int64_t m, n, k;       /* axis lengths */
   int64_t workspace_bytes;
+  int32_t paired_loads;  /* 1: the gathers load two adjacent elements per lane (real,
+                            one storage type, tiles up to 64 x 128, spans below 2 GiB) */
+  int32_t _pad;
 } qs_plan_info;
 
 /* The gfx950 TargetProfile (counterpart of src/hardware/target.jl:26-47). */

@@ -284,6 +284,19 @@ qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
     kB[k] = p->kB_lo + (int64_t(1) << 31) / elem_bytes(d.b_type);
   }
   p->c_mfast = (M > 1 && mC[1] - mC[0] == 1) || !(N > 1 && nC[1] - nC[0] == 1);
+  {
+    // Paired loads read entries 2i and 2i + 1 of the table each gather is
+    // laid along; both must be adjacent in memory for every i.
+    auto pairs = [](const int64_t *t, int64_t n) {
+      if (n % 2 != 0) return false;
+      for (int64_t i = 0; i < n; i += 2)
+        if (t[i + 1] != t[i] + 1) return false;
+      return true;
+    };
+    const bool va = p->a_kfast ? pairs(kA, K) : pairs(mA, M);
+    const bool vb = p->b_kfast ? pairs(kB, K) : pairs(nB, N);
+    p->vec = (va && vb) ? 1 : 0;
+  }
   hipError_t e = hipSuccess;
   if (nt > 0) {
     e = hipMalloc(&p->d_tables, sizeof(int64_t) * (size_t)nt);
@@ -435,6 +448,7 @@ GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *
   g.a_bytes = buf ? (uint32_t)a_bytes : 0;
   g.b_bytes = buf ? (uint32_t)b_bytes : 0;
   g.mixed = buf ? 0 : 1;
+  g.vec = buf && !is_complex(d.compute_type) ? p->vec : 0;
   g.batch_off = nullptr;
   g.batch = 1;
   return g;
@@ -529,6 +543,9 @@ qs_status qs_plan_get_info(const qs_plan *p, qs_plan_info *out) {
   out->n = p->N;
   out->k = p->K;
   out->workspace_bytes = p->workspace_bytes;
+  out->paired_loads =
+      p->path == QS_PATH_GEMM && !p->mixed && !is_complex(p->desc.compute_type) && ts.bm * ts.bn <= 64 * 128 ? p->vec : 0;
+  out->_pad = 0;
   return QS_OK;
 }
 

@@ -223,16 +223,24 @@ __device__ const double qs_zero16[2] = {0.0, 0.0};
 // t/BK + r*(256/BK).  Row offsets are read once; K offsets are fetched one
 // stage ahead of the data that uses them.  Loads are unconditional at
 // clamped in-range coordinates; out-of-tile values become zero.
-template <typename R, bool CPLX, bool MIXED, int ROWS, int BK, bool KF, int VS = 2>
+//
+// VW == 2 (real, one storage type): each load takes two consecutive elements
+// -- two rows of one K step (KF == false) or two K steps of one row -- that
+// the host has found adjacent in memory for every such pair of the operand's
+// tables, in one 2 x ESZ-byte buffer load.  Half the load instructions.
+template <typename R, bool CPLX, bool MIXED, int ROWS, int BK, bool KF, int VS = 2, int VW = 1>
 struct Gather {
   static constexpr int P = CPLX ? 2 : 1;
-  static constexpr int NL = (ROWS * BK + 255) / 256;
+  static constexpr int NL = (ROWS * BK + 256 * VW - 1) / (256 * VW);   // loads per thread
+  static constexpr int RU = KF ? ROWS : ROWS / VW;   // row units (a unit is VW rows when KF == false)
+  static constexpr int KU = KF ? BK / VW : BK;       // K units (VW K steps when KF == true)
   static constexpr int NR = KF ? NL : 1;           // distinct rows per thread
   static constexpr int NK = KF ? 1 : NL;           // distinct K steps per thread
-  static constexpr int RSTEP = KF ? 256 / BK : 0;  // row step between loads
-  static constexpr int KSTEP = KF ? 0 : 256 / ROWS;
-  static constexpr bool UNIFORM_K = !KF && ROWS >= 64;
-  static constexpr bool PARTIAL = ROWS * BK % 256 != 0;
+  static constexpr int RSTEP = KF ? 256 / KU : 0;  // row step between loads
+  static constexpr int KSTEP = KF ? 0 : 256 / RU;
+  static constexpr bool UNIFORM_K = !KF && RU >= 64;
+  static constexpr bool PARTIAL = ROWS * BK % (256 * VW) != 0;
+  static_assert(VW == 1 || (!CPLX && !MIXED && !PARTIAL), "paired loads: real, one storage type, whole stages");
   // One storage type: buffer loads over the operand's accessed span, with
   // 32-bit byte offsets and no bounds tests.  Rows past M (N) repeat the last
   // row: they only reach accumulators of rows (columns) that are never
@@ -254,14 +262,14 @@ struct Gather {
   int row0, kk0;
   Off koff[KS][NK];
   bool kok[KS][NK];
-  R val[VS][NL][P];   // VS stages in flight
+  R val[VS][NL][P * VW];   // VS stages in flight
 
   // `lo` / `bytes`: the operand's accessed span (elements from its storage
   // start, and its length in bytes) for the buffer loads.
   __device__ __forceinline__ void init(int t, int64_t r0, int64_t rows_total, int64_t base,
                                        const int64_t *__restrict__ rowtab, const void *p, int64_t lo, uint32_t bytes) {
-    row0 = KF ? t / BK : t % ROWS;
-    kk0 = KF ? t % BK : t / ROWS;
+    row0 = KF ? t / KU : VW * (t % RU);
+    kk0 = KF ? VW * (t % KU) : t / RU;
     if (UNIFORM_K) kk0 = __builtin_amdgcn_readfirstlane(kk0);
     if constexpr (BUF)
       rsrc = __builtin_amdgcn_make_buffer_rsrc((void *)((const char *)p + lo * ESZ), (short)0, (int)bytes, 0x00020000);
@@ -269,7 +277,8 @@ struct Gather {
     for (int r = 0; r < NR; ++r) {
       const int64_t gi = r0 + row0 + r * RSTEP;
       rowok[r] = gi < rows_total && (!PARTIAL || t + 256 * r < ROWS * BK);
-      const int64_t o = base + rowtab[gi < rows_total ? gi : rows_total - 1];
+      // Past the last row: the last one (the last pair; rows_total is even when VW == 2).
+      const int64_t o = base + rowtab[gi < rows_total ? gi : rows_total - (KF ? 1 : VW)];
       // Row and K byte offsets add up, modulo 2^32, to the element's offset in the span.
       rowoff[r] = BUF ? (Off)(uint32_t)((o - lo) * ESZ) : (Off)o;
     }
@@ -302,7 +311,16 @@ struct Gather {
         // The loaded vector is cast whole to a scalar or a struct before any
         // part is taken: extracting its lanes directly lets the compiler
         // narrow the load to one dword (and then read the wrong lane).
-        if constexpr (ESZ == 4) {
+        if constexpr (VW == 2) {
+          struct Two { R x, y; };
+          Two v;
+          if constexpr (ESZ == 4)
+            v = __builtin_bit_cast(Two, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
+          else
+            v = __builtin_bit_cast(Two, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+          val[S][r][0] = v.x;
+          val[S][r][1] = v.y;
+        } else if constexpr (ESZ == 4) {
           val[S][r][0] = __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0));
         } else if constexpr (ESZ == 8 && !CPLX) {
           val[S][r][0] = __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
@@ -334,8 +352,13 @@ struct Gather {
       if (!PARTIAL || t + 256 * r < ROWS * BK) {
         const int row = row0 + (KF ? r * RSTEP : 0);
         const int k = kk0 + (KF ? 0 : r * KSTEP);
-        lds[L::at(row, k)] = val[S][r][0];
-        if (CPLX) lds[PLANE + L::at(row, k)] = sgn * val[S][r][P - 1];
+        if constexpr (VW == 2) {
+          lds[L::at(row, k)] = val[S][r][0];
+          lds[L::at(KF ? row : row + 1, KF ? k + 1 : k)] = val[S][r][1];
+        } else {
+          lds[L::at(row, k)] = val[S][r][0];
+          if (CPLX) lds[PLANE + L::at(row, k)] = sgn * val[S][r][P - 1];
+        }
       }
     }
   }
@@ -365,7 +388,7 @@ __device__ __forceinline__ void sched_interleave() {
 // One BM x BN tile of C over the K range of split `split`: the whole
 // workgroup's work, shared by the one-contraction kernel and the grouped
 // persistent kernel.  `smem` holds SMEM elements of R.
-template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED>
+template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED, bool VEC = false>
 struct TileGemm {
   static constexpr int P = CPLX ? 2 : 1;
   static constexpr int BK = pick_bk<R, CPLX, BM, BN, AKF, BKF>();
@@ -419,8 +442,8 @@ struct TileGemm {
       // Wave-uniform offsets (scalar loads) are fetched one stage ahead.
       auto deep = [&](auto ns) {
         constexpr int NS = decltype(ns)::value;
-        Gather<R, CPLX, MIXED, BM, BK, AKF, NS> ga;
-        Gather<R, CPLX, MIXED, BN, BK, BKF, NS> gb;
+        Gather<R, CPLX, MIXED, BM, BK, AKF, NS, VEC ? 2 : 1> ga;
+        Gather<R, CPLX, MIXED, BN, BK, BKF, NS, VEC ? 2 : 1> gb;
         ga.init(t, m0, g.M, g.a_base, g.mA, g.A, g.a_lo, g.a_bytes);
         gb.init(t, n0, g.N, g.b_base, g.nB, g.B, g.b_lo, g.b_bytes);
         typedef decltype(ga) GA;
@@ -703,9 +726,9 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const GemmArgs g) {
   }
 }
 
-template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED>
+template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED, bool VEC = false>
 __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmArgs g) {
-  typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, MIXED> TG;
+  typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, MIXED, VEC> TG;
   __shared__ R smem[TG::SMEM];
   const int64_t tiles_m = (g.M + BM - 1) / BM;
   const int64_t tiles_n = (g.N + BN - 1) / BN;
@@ -761,6 +784,25 @@ static hipError_t launch_tile_mixed(const GemmArgs &g, hipStream_t s) {
     }
   }
   dim3 grid((unsigned)tiles, (unsigned)g.k_splits), block(256);
+  // Paired loads: real, one storage type, tiles on the two-stage pipeline.
+  if constexpr (!MIXED && !CPLX && BM * BN <= 64 * 128) {
+    if (g.vec) {
+      if (g.a_kfast && g.b_kfast)
+        hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, true, true, false, true>), grid, block, 0, s, g);
+      else if (g.a_kfast)
+        hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, true, false, false, true>), grid, block, 0, s, g);
+      else if (g.b_kfast)
+        hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, false, true, false, true>), grid, block, 0, s, g);
+      else
+        hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, false, false, false, true>), grid, block, 0, s, g);
+      if (g.k_splits > 1) {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((splitk_reduce_kernel<R, CPLX, MIXED, BM, BN>), dim3((unsigned)tiles), block, 0, s, g);
+      }
+      return hipGetLastError();
+    }
+  }
   if (g.a_kfast && g.b_kfast)
     hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, true, true, MIXED>), grid, block, 0, s, g);
   else if (g.a_kfast)

@@ -59,6 +59,9 @@ struct GemmArgs {
   // batch_off null: one contraction.
   const int64_t *batch_off;
   int64_t batch;
+  // Every row pair (K pair) the gathers read in twos is adjacent in memory:
+  // the tiles on the two-stage pipeline load two elements per lane.
+  int vec = 0;
 };
 
 // Row pitch, in elements, of an LDS tile stored [row][k]: the smallest pitch

@@ -17,6 +17,7 @@ struct qs_plan {
   int a_kfast = 0, b_kfast = 0;
   int c_mfast = 1;             // C's unit-stride side: M (1) or N (0)
   int mixed = 0;
+  int vec = 0;                // the gathers' element pairs are adjacent (GemmArgs::vec)
   int moduli = 0;             // split precision: CRT moduli (INT8 GEMMs)
   int64_t workspace_bytes = 0;
   // Device offset tables: mA mC nB nC kA kB, one allocation.

@@ -394,6 +394,43 @@ def test_every_tile_shape(oracle, tile):
                       path_modes=qs.PathModes("never", "never"))
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("akf", [False, True])
+def test_paired_loads_match_single_loads(oracle, dtype, akf):
+    """Tiles on the two-stage pipeline load two adjacent elements per lane when
+    every pair the gathers read is adjacent.  Every tile sums K in the same
+    order, so their results match the 128 x 128 tile's (one element per lane)
+    bit for bit, partial tiles included; an odd M (for an M-fast A) or K
+    strided by 2 turns the pairing off."""
+    rng = np.random.default_rng(90 + int(akf))
+    M, K, N = 250, 98, 190
+    A = randn(rng, dtype, 2 * M * K)
+    B = randn(rng, dtype, 2 * K * N)
+    C0 = randn(rng, dtype, M * N)
+
+    def views(m, kstride):
+        a_st = (K * kstride, kstride) if akf else (1, m)   # A K-fast (rows along K) or M-fast
+        Av = dev_view(A, (m, K), a_st)
+        Bv = dev_view(B, (K, N), (kstride, K * kstride))     # B K-fast
+        Cv = dev_view(C0, (m, N), (1, m))
+        return Av, Bv, Cv
+
+    # An odd M breaks the row pairs of an M-fast A only; a K-fast A pairs along K.
+    for m, kstride, paired in ((M, 1, True), (M - 1, 1, akf), (M, 2, False)):
+        case(f"paired loads {np.dtype(dtype).name}, A {'K' if akf else 'M'}-fast, M={m}, K stride {kstride}")
+        results = []
+        for tile in range(5):
+            Av, Bv, Cv = views(m, kstride)
+            plan = run_and_check(oracle, Cv, Av, (1, 2), Bv, (2, 3), (1, 3), 1.5, 0.5,
+                                 kernel=qs.MFMAKernel(tile=tile, k_splits=1), path_modes=qs.PathModes("never", "never"))
+            assert plan.info["k_splits"] == 1, plan
+            on = plan.info["paired_loads"] == 1
+            assert on == (paired and plan.info["tile_m"] * plan.info["tile_n"] <= 64 * 128), (plan, plan.info)
+            results.append(values(Cv).copy())
+        for r in results[1:]:
+            assert np.array_equal(r, results[0])
+
+
 def test_host_memory_operands(oracle):
     case("host-memory operands staged through the library")
     rng = np.random.default_rng(12)
