@@ -144,6 +144,33 @@ typedef struct qs_target_profile {
   double peak_fp64_tflops, peak_fp32_tflops, peak_int8_tops, peak_hbm_gbs;
 } qs_target_profile;
 
+/* ---- workspace allocator --------------------------------------------------
+ * Counterpart of plan_contract's `allocator` (src/planning/plan.jl:125-131)
+ * and of ContractWorkspace's tensoralloc / release! (src/execution/
+ * workspace.jl:83-123): the plan's device workspace -- split-K partial sums
+ * and the split-precision residues, scales and samples -- comes from the
+ * caller's allocator and goes back to it in reverse acquisition order, on
+ * qs_plan_release or qs_plan_destroy.  The offset tables stay the library's,
+ * as the reference's offset vectors are plain Vectors.  A Julia caller wraps
+ * a TensorOperations allocator over device arrays in two @cfunction
+ * callbacks; the Python mirror bridges torch's caching allocator. */
+typedef struct qs_allocator {
+  /* `bytes` of device memory, aligned to 256 bytes, or NULL. */
+  void *(*alloc)(void *ctx, size_t bytes);
+  void (*free)(void *ctx, void *ptr, size_t bytes);
+  void *ctx;
+} qs_allocator;
+
+/* qs_plan_create with the workspace from `allocator` (NULL: the library's). */
+qs_status qs_plan_create_with_allocator(const qs_contract_desc *desc, const qs_allocator *allocator,
+                                        qs_plan **plan);
+
+/* release!(plan, allocator): hand the workspace back now, split-precision
+ * buffer first, then the split-K partials.  The plan's info stays readable;
+ * qs_execute and qs_autotune on it then fail with QS_ERR_ARGUMENT, and
+ * qs_plan_destroy frees the rest. */
+qs_status qs_plan_release(qs_plan *plan);
+
 /* Last error message of the calling thread ("" if none). */
 const char *qs_last_error(void);
 

@@ -19,6 +19,7 @@ from .target import target_profile
 from .group import GroupPlan, Graph
 from .batch import BatchedPlan, batch_labels, contract_batched, plan_batched
 from .server import Server
+from .allocators import DefaultAllocator, TorchAllocator
 from . import _lib
 
 __all__ = [
@@ -29,5 +30,5 @@ __all__ = [
     "plan_contract", "planned", "release", "QuasiStridedBackend", "contraction_labels", "tensorcontract",
     "tensoradd", "tensortrace",
     "target_profile", "GroupPlan", "Graph", "Server", "BatchedPlan", "batch_labels", "contract_batched",
-    "plan_batched",
+    "plan_batched", "DefaultAllocator", "TorchAllocator",
 ]

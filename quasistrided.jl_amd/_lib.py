@@ -58,6 +58,14 @@ class QSPlanInfo(C.Structure):
     ]
 
 
+QS_ALLOC_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_size_t)
+QS_FREE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_size_t)
+
+
+class QSAllocator(C.Structure):
+    _fields_ = [("alloc", QS_ALLOC_FN), ("free", QS_FREE_FN), ("ctx", C.c_void_p)]
+
+
 class QSTuneResult(C.Structure):
     _fields_ = [("tile", C.c_int32), ("k_splits", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double),
                 ("candidates", C.c_int32), ("_pad", C.c_int32)]
@@ -108,6 +116,8 @@ SIGNATURES = {
     "qs_plan_create": (C.c_int, [C.POINTER(QSContractDesc), C.POINTER(C.c_void_p)]),
     "qs_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "qs_plan_get_info": (C.c_int, [C.c_void_p, C.POINTER(QSPlanInfo)]),
+    "qs_plan_create_with_allocator": (C.c_int, [C.c_void_p, C.POINTER(QSAllocator), C.POINTER(C.c_void_p)]),
+    "qs_plan_release": (C.c_int, [C.c_void_p]),
     "qs_plan_rebind": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
                                  C.c_void_p, C.c_int64, C.c_int64]),
     "qs_plan_destroy": (C.c_int, [C.c_void_p]),

@@ -76,11 +76,15 @@ class QuasiStridedBackend:
         compute_type(Av.dtype, Bv.dtype, Cv.dtype, self.accumulator)
         _argcheck(Cv, Av, pA, Bv, pB, pAB)
         indA, indB, indC = contraction_labels(pA, pB, pAB)
-        plan = planned(Cv, Av, indA, Bv, indB, indC, conjA=conjA, conjB=conjB, accumulator=self.accumulator)
+        checkpoint = allocator.checkpoint() if callable(getattr(allocator, "checkpoint", None)) else None
+        plan = planned(Cv, Av, indA, Bv, indB, indC, conjA=conjA, conjB=conjB, accumulator=self.accumulator,
+                       allocator=allocator)
         try:
             execute(plan, alpha, beta)
         finally:
             plan.release()
+            if checkpoint is not None:
+                allocator.reset(checkpoint)
         return C
 
     def tensoradd(self, C, A, pA, conjA, alpha, beta, allocator=None):
@@ -96,5 +100,5 @@ class QuasiStridedBackend:
         return _add.tensortrace(C, A, p, q, conjA, alpha, beta)
 
 
-def tensorcontract(C, A, pA, conjA, B, pB, conjB, pAB, alpha, beta, backend):
-    return backend.tensorcontract(C, A, pA, conjA, B, pB, conjB, pAB, alpha, beta)
+def tensorcontract(C, A, pA, conjA, B, pB, conjB, pAB, alpha, beta, backend, allocator=None):
+    return backend.tensorcontract(C, A, pA, conjA, B, pB, conjB, pAB, alpha, beta, allocator)

@@ -182,7 +182,7 @@ static double scalar_at(const void *p, int ct, int part) {
   return ((const double *)p)[part];
 }
 
-qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
+qs_status plan_build(const qs_contract_desc *desc, const qs_allocator *al, qs_plan **out) {
   if (!desc || !out) return fail(QS_ERR_ARGUMENT, "qs_plan_create: null argument");
   const qs_contract_desc &d = *desc;
   for (int t : {d.a_type, d.b_type, d.c_type, d.compute_type})
@@ -204,6 +204,10 @@ qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
     return fail(QS_ERR_ARGUMENT, "plan_contract: negative storage length");
 
   qs_plan *p = new qs_plan();
+  if (al) {
+    p->al = *al;
+    p->has_al = 1;
+  }
   p->desc = d;
   p->M = group_length(d.m);
   p->N = group_length(d.n);
@@ -339,6 +343,28 @@ qs_status plan_build(const qs_contract_desc *desc, qs_plan **out) {
   return QS_OK;
 }
 
+void *ws_alloc(qs_plan *p, size_t bytes) {
+  if (p->has_al) {
+    void *q = p->al.alloc(p->al.ctx, bytes);
+    if (q && (uintptr_t)q % 256 != 0) {   // misaligned: handed back unused
+      p->al.free(p->al.ctx, q, bytes);
+      q = nullptr;
+    }
+    return q;
+  }
+  void *q = nullptr;
+  if (hipMalloc(&q, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return q;
+}
+void ws_free(qs_plan *p, void *ptr, size_t bytes) {
+  if (!ptr) return;
+  if (p->has_al) p->al.free(p->al.ctx, ptr, bytes);
+  else (void)hipFree(ptr);
+}
+
 qs_status configure_gemm(qs_plan *p, int tile, int splits) {
   const int ct = p->desc.compute_type;
   const TileShape ts = tile_shape(tile, ct, p->a_kfast, p->b_kfast);
@@ -356,17 +382,16 @@ qs_status configure_gemm(qs_plan *p, int tile, int splits) {
   const size_t need = ks > 1 ? (size_t)ks * (is_complex(ct) ? 2 : 1) * (size_t)p->M * (size_t)p->N *
                                    bytes_of_compute_real(ct)
                              : 0;
-  if (need > (size_t)p->workspace_bytes) {
-    if (p->d_partials) (void)hipFree(p->d_partials);
+  if (need > (size_t)p->partials_bytes) {
+    if (p->d_partials) ws_free(p, p->d_partials, (size_t)p->partials_bytes);
     p->d_partials = nullptr;
-    p->workspace_bytes = 0;
-    if (hipMalloc(&p->d_partials, need) != hipSuccess) {
-      (void)hipGetLastError();
-      p->d_partials = nullptr;
+    p->partials_bytes = 0;
+    p->d_partials = ws_alloc(p, need);
+    if (!p->d_partials) {
       ks = 1;   // no room for partial sums: run unsplit
       chunk = p->K;
     } else {
-      p->workspace_bytes = (int64_t)need;
+      p->partials_bytes = (int64_t)need;
     }
   }
   p->tile = tile;
@@ -524,7 +549,27 @@ qs_status qs_target_profile_get(qs_target_profile *out) {
 
 qs_status qs_plan_create(const qs_contract_desc *desc, qs_plan **out) {
   if (out) *out = nullptr;
-  return plan_build(desc, out);
+  return plan_build(desc, nullptr, out);
+}
+
+qs_status qs_plan_create_with_allocator(const qs_contract_desc *desc, const qs_allocator *al, qs_plan **out) {
+  if (out) *out = nullptr;
+  if (al && (!al->alloc || !al->free)) return fail(QS_ERR_ARGUMENT, "plan_contract: allocator without alloc / free");
+  return plan_build(desc, al, out);
+}
+
+
+qs_status qs_plan_release(qs_plan *p) {
+  if (!p) return fail(QS_ERR_ARGUMENT, "release!: null plan");
+  if (p->released) return QS_OK;
+  // Work already queued on the workspace finishes before it is handed back.
+  if (hipDeviceSynchronize() != hipSuccess) return fail(QS_ERR_DEVICE, "release!: device sync");
+  ws_free(p, p->d_split, (size_t)p->split_bytes);
+  ws_free(p, p->d_partials, (size_t)p->partials_bytes);
+  p->d_split = nullptr;
+  p->d_partials = nullptr;
+  p->released = 1;
+  return QS_OK;
 }
 
 qs_status qs_plan_get_info(const qs_plan *p, qs_plan_info *out) {
@@ -542,7 +587,7 @@ qs_status qs_plan_get_info(const qs_plan *p, qs_plan_info *out) {
   out->m = p->M;
   out->n = p->N;
   out->k = p->K;
-  out->workspace_bytes = p->workspace_bytes;
+  out->workspace_bytes = p->released ? 0 : p->partials_bytes + p->split_bytes;
   out->paired_loads =
       p->path == QS_PATH_GEMM && !p->mixed && !is_complex(p->desc.compute_type) && ts.bm * ts.bn <= 64 * 128 ? p->vec : 0;
   out->_pad = 0;
@@ -551,6 +596,7 @@ qs_status qs_plan_get_info(const qs_plan *p, qs_plan_info *out) {
 
 qs_status qs_execute(qs_plan *p, const void *alpha, const void *beta, void *stream) {
   if (!p || !alpha || !beta) return fail(QS_ERR_ARGUMENT, "execute!: null argument");
+  if (p->released) return fail(QS_ERR_ARGUMENT, "execute!: the plan's workspace was released");
   const int ct = p->desc.compute_type;
   double al[2] = {scalar_at(alpha, ct, 0), is_complex(ct) ? scalar_at(alpha, ct, 1) : 0.0};
   double be[2] = {scalar_at(beta, ct, 0), is_complex(ct) ? scalar_at(beta, ct, 1) : 0.0};
@@ -578,9 +624,13 @@ qs_status qs_plan_rebind(qs_plan *p, const void *a, int64_t a_base, int64_t a_le
 
 qs_status qs_plan_destroy(qs_plan *p) {
   if (!p) return QS_OK;
+  if (!p->released && (p->d_split || p->d_partials)) {
+    // Queued work may still read the workspace; an allocator may hand it on at once.
+    if (p->has_al) (void)hipDeviceSynchronize();
+    ws_free(p, p->d_split, (size_t)p->split_bytes);
+    ws_free(p, p->d_partials, (size_t)p->partials_bytes);
+  }
   if (p->d_tables) (void)hipFree(p->d_tables);
-  if (p->d_partials) (void)hipFree(p->d_partials);
-  if (p->d_split) (void)hipFree(p->d_split);
   if (p->split_side) (void)hipStreamDestroy(p->split_side);
   if (p->split_fork) (void)hipEventDestroy(p->split_fork);
   if (p->split_join) (void)hipEventDestroy(p->split_join);

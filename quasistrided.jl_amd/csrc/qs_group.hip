@@ -367,6 +367,7 @@ extern "C" {
 
 static qs_status check_member(const qs_plan *p, int32_t i, int32_t tile) {
   if (!p) return fail(QS_ERR_ARGUMENT, "qs_group_create: null plan " + std::to_string(i));
+  if (p->released) return fail(QS_ERR_ARGUMENT, "qs_group_create: plan " + std::to_string(i) + " was released");
   if (p->desc.memory != QS_MEM_DEVICE)
     return fail(QS_ERR_ARGUMENT, "qs_group_create: plan " + std::to_string(i) + " is not in device memory");
   if (p->path != QS_PATH_GEMM && p->path != QS_PATH_SKINNY)

@@ -19,11 +19,16 @@ struct qs_plan {
   int mixed = 0;
   int vec = 0;                // the gathers' element pairs are adjacent (GemmArgs::vec)
   int moduli = 0;             // split precision: CRT moduli (INT8 GEMMs)
-  int64_t workspace_bytes = 0;
+  int64_t partials_bytes = 0;   // capacity of d_partials (split-K partial sums)
   // Device offset tables: mA mC nB nC kA kB, one allocation.
   int64_t *d_tables = nullptr;
   const int64_t *t_mA = nullptr, *t_mC = nullptr, *t_nB = nullptr, *t_nC = nullptr, *t_kA = nullptr, *t_kB = nullptr;
   void *d_partials = nullptr;
+  // The workspace's allocator (the library's when `has_al` is 0), and whether
+  // the workspace was handed back (qs_plan_release).
+  qs_allocator al = {nullptr, nullptr, nullptr};
+  int has_al = 0;
+  int released = 0;
   // Map ranges (exact min / max offsets of each map).
   int64_t mA_lo = 0, mA_hi = 0, mC_lo = 0, mC_hi = 0, nB_lo = 0, nB_hi = 0, nC_lo = 0, nC_hi = 0;
   int64_t kA_lo = 0, kA_hi = 0, kB_lo = 0, kB_hi = 0;
@@ -44,6 +49,9 @@ struct qs_plan {
 };
 
 namespace qs {
+// The plan's device workspace, from its allocator (or hipMalloc / hipFree).
+void *ws_alloc(qs_plan *p, size_t bytes);
+void ws_free(qs_plan *p, void *ptr, size_t bytes);
 // Device and pinned host buffers of one in-flight grouped batch.
 struct BatchBuffers {
   void *d_items = nullptr, *h_items = nullptr;

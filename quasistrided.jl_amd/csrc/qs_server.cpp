@@ -214,6 +214,7 @@ qs_status qs_server_submit(qs_server *s, qs_plan *p, const void *a, int64_t a_ba
                            const double *beta, uint64_t *ticket) {
   if (!s || !p || !ticket) return fail(QS_ERR_ARGUMENT, "qs_server_submit: null argument");
   if (p->desc.memory != QS_MEM_DEVICE) return fail(QS_ERR_ARGUMENT, "qs_server_submit: device-memory plans only");
+  if (p->released) return fail(QS_ERR_ARGUMENT, "qs_server_submit: the plan was released");
   if (p->path != QS_PATH_GEMM && p->path != QS_PATH_SKINNY)
     return fail(QS_ERR_ARGUMENT, "qs_server_submit: the plan does not run the tiled GEMM path");
   if (!a) { a = p->desc.a; a_base = p->desc.a_base; a_len = p->desc.a_len; }

@@ -762,13 +762,13 @@ qs_status split_prepare(qs_plan *p) {
   p->split_Np = rup(p->N, kTile);
   p->split_Kp = rup(std::max<int64_t>(p->K, 1), 256);   // a multiple of four 64-K stages (the K loop's unroll)
   const SplitLayout L = split_layout(p);
-  if (hipMalloc(&p->d_split, L.total) != hipSuccess) return fail(QS_ERR_MEMORY, "plan_contract: split workspace");
+  p->d_split = ws_alloc(p, L.total);
+  if (!p->d_split) return fail(QS_ERR_MEMORY, "plan_contract: split workspace");
+  p->split_bytes = (int64_t)L.total;
   if (hipStreamCreateWithFlags(&p->split_side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&p->split_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&p->split_join, hipEventDisableTiming) != hipSuccess)
     return fail(QS_ERR_DEVICE, "plan_contract: split side stream");
-  p->split_bytes = (int64_t)L.total;
-  p->workspace_bytes += (int64_t)L.total;
   return QS_OK;
 }
 

@@ -73,6 +73,7 @@ extern "C" {
 
 qs_status qs_autotune(qs_plan *p, void *stream, int32_t reps, qs_tune_result *out) {
   if (!p) return fail(QS_ERR_ARGUMENT, "autotune: null plan");
+  if (p->released) return fail(QS_ERR_ARGUMENT, "autotune: the plan's workspace was released");
   if (out) std::memset(out, 0, sizeof *out);
   if (p->path != QS_PATH_GEMM && p->path != QS_PATH_SKINNY) {
     if (out) { out->tile = p->tile; out->k_splits = p->k_splits; }

@@ -15,6 +15,7 @@ from dataclasses import dataclass
 import torch
 
 from . import _lib
+from . import allocators as _alloc
 from .errors import ArgumentError, raise_status
 from .labels import (AxisGroup, axis_length, classify_labels, group_from_labels, leading_unit_run,
                      order_contract_labels, order_free_labels, prefer_swap)
@@ -120,6 +121,7 @@ class ContractPlan:
         self.T = T
         self.swapped = swapped
         self._views = views  # keeps every operand alive while the plan lives
+        self._al_keep = None  # the allocator bridge, kept until the plan is destroyed
         info = _lib.QSPlanInfo()
         st = _lib.lib().qs_plan_get_info(handle, C.byref(info))
         if st != 0:
@@ -137,9 +139,19 @@ class ContractPlan:
         return self._handle
 
     def release(self):
+        """Free the plan (its workspace goes back to its allocator first)."""
         if self._handle is not None:
             _lib.lib().qs_plan_destroy(self._handle)
             self._handle = None
+        self._al_keep = None
+
+    def release_workspace(self):
+        """`release!(plan, allocator)` (workspace.jl:118-123): hand the device
+        workspace back to the allocator now; the plan cannot execute after."""
+        st = _lib.lib().qs_plan_release(self.handle)
+        if st != 0:
+            raise_status(st, _lib.last_error())
+        _refresh_info(self)
 
     def __del__(self):
         try:
@@ -222,7 +234,7 @@ def _fill_group(g: AxisGroup) -> _lib.QSGroup:
 
 def planned(C_, A_, indA, B_, indB, indC, *, kernel=None, conjA=False, conjB=False, m_block=None,
             k_block=None, n_block=None, accumulator=None, path_modes=None, precision="plain",
-            target_error=0.0) -> ContractPlan:
+            target_error=0.0, allocator=None) -> ContractPlan:
     """`planned` (plan.jl:197-262) up to the device plan."""
     Cv, Av, Bv = StridedView.of(C_), StridedView.of(A_), StridedView.of(B_)
     indA, indB, indC = tuple(indA), tuple(indB), tuple(indC)
@@ -283,11 +295,18 @@ def planned(C_, A_, indA, B_, indB, indC, *, kernel=None, conjA=False, conjB=Fal
     d.k_splits = int(getattr(kern, "k_splits", 0))
     d.path_mode = 1 if (modes.dot == "never" and modes.outer == "never") else 0
     handle = C.c_void_p()
-    st = _lib.lib().qs_plan_create(C.byref(d), C.byref(handle))
+    keep = None
+    if _alloc.is_default(allocator):
+        st = _lib.lib().qs_plan_create(C.byref(d), C.byref(handle))
+    else:
+        al, keep = _alloc.bridge(allocator)
+        st = _lib.lib().qs_plan_create_with_allocator(C.byref(d), C.byref(al), C.byref(handle))
     if st != 0:
         raise_status(st, _lib.last_error())
-    return ContractPlan(handle, kern, mgroup, ngroup, kgroup, Av.parent, Av.offset, Bv.parent, Bv.offset,
+    plan = ContractPlan(handle, kern, mgroup, ngroup, kgroup, Av.parent, Av.offset, Bv.parent, Bv.offset,
                         Cv.parent, Cv.offset, atransform, btransform, T, swapped, (Av, Bv, Cv))
+    plan._al_keep = keep
+    return plan
 
 
 def plan_contract(C_, A_, indA, B_, indB, indC, *, kernel=None, conjA=False, conjB=False, m_block=None,
@@ -296,11 +315,12 @@ def plan_contract(C_, A_, indA, B_, indB, indC, *, kernel=None, conjA=False, con
     """Plan `C[indC] = A[indA] * B[indB]` (plan.jl:115-133).  Operands are
     StridedViews or torch tensors (a lazily conjugated tensor is a `conj` op).
     `m_block`/`k_block`/`n_block` are validated as the reference does; the
-    device picks its own workgroup tile.  `allocator` is accepted for
-    signature parity (device memory comes from the library)."""
+    device picks its own workgroup tile.  `allocator` supplies the plan's
+    device workspace (see `allocators`); None or DefaultAllocator: the
+    library's."""
     return planned(C_, A_, indA, B_, indB, indC, kernel=kernel, conjA=conjA, conjB=conjB, m_block=m_block,
                    k_block=k_block, n_block=n_block, accumulator=accumulator, path_modes=path_modes,
-                   precision=precision, target_error=target_error)
+                   precision=precision, target_error=target_error, allocator=allocator)
 
 
 def _scalar_buffer(x, T: torch.dtype):
