@@ -137,6 +137,34 @@ def bench_split(qs, torch, dev, d=64, reps=10):
     return out
 
 
+def bench_mixed_domain(qs, torch, dev, reps=50):
+    """Complex A times real B (ComplexRealKernel's case) and real A times
+    complex B, on the permuted all-32 workload (ComplexF64 / Float64 storages,
+    ComplexF64 C), against the all-complex contraction of the same shapes.
+    Useful flops: 4 M N K for a complex-real product (8 M N K for complex)."""
+    out = {}
+    rng = np.random.default_rng(9)
+    d = 32
+    SV = qs.StridedView
+    pst = colmajor((d, d, 2 * d, d))
+    ast = (pst[0], pst[1], 2 * pst[2], pst[3])
+    for name, ta, tb in (("c64_x_f64", np.complex128, np.float64), ("f64_x_c64", np.float64, np.complex128),
+                         ("c64_x_c64", np.complex128, np.complex128)):
+        A = SV(torch.from_numpy(randn(rng, ta, d * d * 2 * d * d)).to(dev), (d,) * 4, ast)
+        B = SV(torch.from_numpy(randn(rng, tb, d ** 4)).to(dev), (d,) * 4, colmajor((d,) * 4))
+        C = SV(torch.from_numpy(randn(rng, np.complex128, d ** 4)).to(dev), (d,) * 4, colmajor((d,) * 4))
+        plan = qs.plan_contract(C, A, (1, 2, 3, 4), B, (4, 5, 2, 6), (1, 3, 5, 6))
+        qs.autotune(plan, reps=3)
+        _, km, kmed = time_plan(qs, torch, plan, 0.5, reps, 5)
+        fl = (8 if name == "c64_x_c64" else 4) * 1024 ** 3
+        out[name] = {"kernel_us": round(kmed * 1e6, 1), "useful_tflops": round(fl / kmed / 1e12, 2),
+                     "mfma_frac": round(fl / kmed / PEAK_FP64, 4), "mixed_domain": plan.info["mixed_domain"],
+                     "plan": repr(plan)}
+        plan.release()
+    out["c64_x_f64_vs_c64_x_c64_time"] = round(out["c64_x_c64"]["kernel_us"] / out["c64_x_f64"]["kernel_us"], 3)
+    return out
+
+
 def bench_batched(qs, torch, dev, reps=20):
     """Batch labels: C[i,j,b] := A[i,k,b] * B[k,j,b], F64, column-major, as one
     grouped launch (and its HIP graph), against torch.bmm on the same data
@@ -514,7 +542,7 @@ def main():
     ap.add_argument("--tune-cache", default=None,
                     help="autotune cache file: loaded if it exists (no tuning runs), else written after tuning")
     ap.add_argument("--only", default=None,
-                    choices=["split", "grouped_f64", "grouped_c64", "served", "bandwidth", "batched"],
+                    choices=["split", "grouped_f64", "grouped_c64", "served", "bandwidth", "batched", "mixed"],
                     help="only run one of the extra workloads and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy 5 times (PMC byte-counter calibration)")
@@ -547,6 +575,8 @@ def main():
             out = bench_bandwidth(qs, torch, dev)
         elif args.only == "batched":
             out = bench_batched(qs, torch, dev)
+        elif args.only == "mixed":
+            out = bench_mixed_domain(qs, torch, dev)
         elif args.only.startswith("grouped"):
             out = bench_grouped(qs, torch, dev, args.only.endswith("c64"))
         else:
@@ -642,6 +672,7 @@ def main():
         extra["split_rank4_all64_f64"] = bench_split(qs, torch, dev)
         extra["bandwidth_f64"] = bench_bandwidth(qs, torch, dev)
         extra["batched_f64"] = bench_batched(qs, torch, dev)
+        extra["mixed_domain_all32"] = bench_mixed_domain(qs, torch, dev)
         extra["grouped_512_mps_f64"] = bench_grouped(qs, torch, dev, False)
         extra["grouped_512_mps_c64"] = bench_grouped(qs, torch, dev, True)
         pools = served_pools(qs, torch, dev)

@@ -122,7 +122,8 @@ typedef struct qs_plan_info {
   int64_t workspace_bytes;
   int32_t paired_loads;  /* 1: the gathers load two adjacent elements per lane (real,
                             one storage type, tiles up to 64 x 128, spans below 2 GiB) */
-  int32_t _pad;
+  int32_t mixed_domain;  /* 1: complex A x real B, 2: real A x complex B, two MFMAs per
+                            block (ComplexRealKernel / RealComplexKernel); 0 otherwise */
 } qs_plan_info;
 
 /* The gfx950 TargetProfile (counterpart of src/hardware/target.jl:26-47). */

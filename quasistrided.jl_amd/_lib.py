@@ -54,7 +54,7 @@ class QSPlanInfo(C.Structure):
         ("path", C.c_int32), ("tile_m", C.c_int32), ("tile_n", C.c_int32), ("tile_k", C.c_int32),
         ("k_splits", C.c_int32), ("a_kfast", C.c_int32), ("b_kfast", C.c_int32), ("moduli", C.c_int32),
         ("m", C.c_int64), ("n", C.c_int64), ("k", C.c_int64), ("workspace_bytes", C.c_int64),
-        ("paired_loads", C.c_int32), ("_pad", C.c_int32),
+        ("paired_loads", C.c_int32), ("mixed_domain", C.c_int32),
     ]
 
 

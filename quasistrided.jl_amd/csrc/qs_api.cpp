@@ -427,6 +427,14 @@ static qs_status check_spans(qs_plan *p, bool reads_ab) {
   return QS_OK;
 }
 
+// Complex A times real B (1) or real A times complex B (2) on the generic
+// gather; 0 otherwise.
+static int mixed_domain(const qs_contract_desc &d, bool generic) {
+  if (!generic || !is_complex(d.compute_type)) return 0;
+  const bool ac = is_complex(d.a_type), bc = is_complex(d.b_type);
+  return (ac && !bc) ? 1 : (!ac && bc) ? 2 : 0;
+}
+
 GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *B, int64_t b_base, void *C,
                    int64_t c_base) {
   const qs_contract_desc &d = p->desc;
@@ -469,11 +477,19 @@ GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *
   g.b_lo = b_base + p->kB_lo + p->nB_lo;
   const int64_t a_bytes = (p->mA_hi - p->mA_lo + p->kA_hi - p->kA_lo + 1) * elem_bytes(d.a_type);
   const int64_t b_bytes = (p->kB_hi - p->kB_lo + p->nB_hi - p->nB_lo + 1) * elem_bytes(d.b_type);
-  const bool buf = !p->mixed && a_bytes <= kBufferSpanMax && b_bytes <= kBufferSpanMax;
-  g.a_bytes = buf ? (uint32_t)a_bytes : 0;
-  g.b_bytes = buf ? (uint32_t)b_bytes : 0;
+  const bool spans_ok = a_bytes <= kBufferSpanMax && b_bytes <= kBufferSpanMax;
+  const bool buf = !p->mixed && spans_ok;
   g.mixed = buf ? 0 : 1;
   g.vec = buf && !is_complex(d.compute_type) ? p->vec : 0;
+  g.dom = mixed_domain(d, !buf);
+  if (g.dom) {
+    // The complex operand and C in the compute type, the real one in its real type.
+    const int ct = d.compute_type, rt = ct == QS_C64 ? QS_F64 : QS_F32;
+    const int at = g.dom == 1 ? ct : rt, bt = g.dom == 1 ? rt : ct;
+    g.dom_buf = spans_ok && d.a_type == at && d.b_type == bt && d.c_type == ct ? 1 : 0;
+  }
+  g.a_bytes = buf || g.dom_buf ? (uint32_t)a_bytes : 0;
+  g.b_bytes = buf || g.dom_buf ? (uint32_t)b_bytes : 0;
   g.batch_off = nullptr;
   g.batch = 1;
   return g;
@@ -590,7 +606,7 @@ qs_status qs_plan_get_info(const qs_plan *p, qs_plan_info *out) {
   out->workspace_bytes = p->released ? 0 : p->partials_bytes + p->split_bytes;
   out->paired_loads =
       p->path == QS_PATH_GEMM && !p->mixed && !is_complex(p->desc.compute_type) && ts.bm * ts.bn <= 64 * 128 ? p->vec : 0;
-  out->_pad = 0;
+  out->mixed_domain = p->path == QS_PATH_GEMM || p->path == QS_PATH_SKINNY ? mixed_domain(p->desc, p->mixed != 0) : 0;
   return QS_OK;
 }
 

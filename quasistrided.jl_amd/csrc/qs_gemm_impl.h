@@ -388,15 +388,25 @@ __device__ __forceinline__ void sched_interleave() {
 // One BM x BN tile of C over the K range of split `split`: the whole
 // workgroup's work, shared by the one-contraction kernel and the grouped
 // persistent kernel.  `smem` holds SMEM elements of R.
-template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED, bool VEC = false>
+//
+// DOM (complex compute type, mixed storage types): 1 = complex A times real
+// B, 2 = real A times complex B -- the reference's ComplexRealKernel and
+// RealComplexKernel (src/microkernels/kernels.jl:125-160).  The real operand
+// keeps one LDS plane and a real gather, and each block takes two MFMAs
+// (re += ar br, im += ai br; or re += ar br, im += ar bi) instead of four.
+template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED, bool VEC = false, int DOM = 0>
 struct TileGemm {
   static constexpr int P = CPLX ? 2 : 1;
+  static constexpr bool CA = CPLX && DOM != 2, CB = CPLX && DOM != 1;   // complex A / B
+  static constexpr int PA = CA ? 2 : 1, PB = CB ? 2 : 1;
+  static constexpr int NMFMA = CPLX ? (DOM ? 2 : 4) : 1;   // MFMAs per 16 x 16 block and k step
+  static_assert(DOM == 0 || CPLX, "mixed domains: complex compute type");
   static constexpr int BK = pick_bk<R, CPLX, BM, BN, AKF, BKF>();
   static constexpr int WTM = BM / 2, WTN = BN / 2;       // 2 x 2 waves
   static constexpr int FM = WTM / 16, FN = WTN / 16;     // MFMA blocks per wave
   typedef LdsTile<R, BM, BK, AKF> LA;
   typedef LdsTile<R, BN, BK, BKF> LB;
-  static constexpr int SA = P * LA::plane, SB = P * LB::plane;   // one stage of A / B
+  static constexpr int SA = PA * LA::plane, SB = PB * LB::plane;   // one stage of A / B
   static constexpr int SMEM = 2 * (SA + SB);
   static_assert(256 % BM == 0 && 256 % BN == 0, "tile rows");
   static_assert(BK % 4 == 0, "BK");
@@ -488,7 +498,7 @@ struct TileGemm {
           gb.template stage<NX, LB::plane>(Ws + SA, t, sb);
           // The stage's loads, fragment reads and LDS writes spread between
           // its MFMAs (the loads are not pinned ahead of them).
-          sched_interleave<(BK / 4) * FM * FN * (CPLX ? 4 : 1), FM * FN >= 4 ? 6 : 3, FM * FN >= 4 ? 1 : 2>();
+          sched_interleave<(BK / 4) * FM * FN * NMFMA, FM * FN >= 4 ? 6 : 3, FM * FN >= 4 ? 1 : 2>();
           __syncthreads();
         };
         if (kbeg < kend) {
@@ -544,8 +554,8 @@ struct TileGemm {
       deep(std::integral_constant<int, 2>());
     } else {
       // One stage in flight: stage j + 1 is loaded while stage j multiplies.
-      Gather<R, CPLX, MIXED, BM, BK, AKF> ga;
-      Gather<R, CPLX, MIXED, BN, BK, BKF> gb;
+      Gather<R, CA, MIXED, BM, BK, AKF> ga;
+      Gather<R, CB, MIXED, BN, BK, BKF> gb;
       ga.init(t, m0, g.M, g.a_base, g.mA, g.A, g.a_lo, g.a_bytes);
       gb.init(t, n0, g.N, g.b_base, g.nB, g.B, g.b_lo, g.b_bytes);
       int buf = 0;
@@ -574,32 +584,39 @@ struct TileGemm {
         const R *Bs = As + SA;
   #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
-          R af[P][FM], bf[P][FN];
+          R af[PA][FM], bf[PB][FN];
   #pragma unroll
-          for (int p = 0; p < P; ++p) {
+          for (int p = 0; p < PA; ++p)
   #pragma unroll
             for (int a = 0; a < FM; ++a) af[p][a] = As[p * LA::plane + LA::at(wm * WTM + a * 16 + fr, ks * 4 + fk)];
   #pragma unroll
+          for (int p = 0; p < PB; ++p)
+  #pragma unroll
             for (int b = 0; b < FN; ++b) bf[p][b] = Bs[p * LB::plane + LB::at(wn * WTN + b * 16 + fr, ks * 4 + fk)];
-          }
   #pragma unroll
           for (int a = 0; a < FM; ++a)
   #pragma unroll
             for (int b = 0; b < FN; ++b) {
-              if (!CPLX) {
+              if constexpr (!CPLX) {
                 acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+              } else if constexpr (DOM == 1) {   // complex A, real B
+                acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+                acc[1][a][b] = mfma16(af[PA - 1][a], bf[0][b], acc[1][a][b]);
+              } else if constexpr (DOM == 2) {   // real A, complex B
+                acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+                acc[1][a][b] = mfma16(af[0][a], bf[PB - 1][b], acc[1][a][b]);
               } else {
                 acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
-                acc[0][a][b] = mfma16(-af[P - 1][a], bf[P - 1][b], acc[0][a][b]);
-                acc[P - 1][a][b] = mfma16(af[0][a], bf[P - 1][b], acc[P - 1][a][b]);
-                acc[P - 1][a][b] = mfma16(af[P - 1][a], bf[0][b], acc[P - 1][a][b]);
+                acc[0][a][b] = mfma16(-af[PA - 1][a], bf[PB - 1][b], acc[0][a][b]);
+                acc[P - 1][a][b] = mfma16(af[0][a], bf[PB - 1][b], acc[P - 1][a][b]);
+                acc[P - 1][a][b] = mfma16(af[PA - 1][a], bf[0][b], acc[P - 1][a][b]);
               }
             }
         }
         R *Ws = smem + (buf ^ 1) * (SA + SB);
         ga.template stage<0, LA::plane>(Ws, t, sa);
         gb.template stage<0, LB::plane>(Ws + SA, t, sb);
-        sched_interleave<(BK / 4) * FM * FN * (CPLX ? 4 : 1), FM * FN >= 4 ? 6 : 3, FM * FN >= 4 ? 1 : 2>();
+        sched_interleave<(BK / 4) * FM * FN * NMFMA, FM * FN >= 4 ? 6 : 3, FM * FN >= 4 ? 1 : 2>();
         __syncthreads();
         buf ^= 1;
       }
@@ -726,9 +743,9 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const GemmArgs g) {
   }
 }
 
-template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED, bool VEC = false>
+template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED, bool VEC = false, int DOM = 0>
 __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmArgs g) {
-  typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, MIXED, VEC> TG;
+  typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, MIXED, VEC, DOM> TG;
   __shared__ R smem[TG::SMEM];
   const int64_t tiles_m = (g.M + BM - 1) / BM;
   const int64_t tiles_n = (g.N + BN - 1) / BN;
@@ -758,6 +775,21 @@ __global__ __launch_bounds__(256, 2) void gemm_batched_kernel(const GemmArgs g) 
   gz.b_lo += o[1];
   gz.c_base += o[2];
   TG::template run<true>(gz, tm, tn, 0, smem);
+}
+
+// A mixed-domain GEMM (complex compute type R, DOM 1 or 2) over the four
+// gather directions.
+template <typename R, int BM, int BN, bool MIXED, int DOM>
+static void launch_dom(const GemmArgs &g, dim3 grid, hipStream_t s) {
+  const dim3 block(256);
+  if (g.a_kfast && g.b_kfast)
+    hipLaunchKernelGGL((gemm_kernel<R, true, BM, BN, true, true, MIXED, false, DOM>), grid, block, 0, s, g);
+  else if (g.a_kfast)
+    hipLaunchKernelGGL((gemm_kernel<R, true, BM, BN, true, false, MIXED, false, DOM>), grid, block, 0, s, g);
+  else if (g.b_kfast)
+    hipLaunchKernelGGL((gemm_kernel<R, true, BM, BN, false, true, MIXED, false, DOM>), grid, block, 0, s, g);
+  else
+    hipLaunchKernelGGL((gemm_kernel<R, true, BM, BN, false, false, MIXED, false, DOM>), grid, block, 0, s, g);
 }
 
 template <typename R, bool CPLX, int BM, int BN, bool MIXED>
@@ -799,6 +831,29 @@ static hipError_t launch_tile_mixed(const GemmArgs &g, hipStream_t s) {
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((splitk_reduce_kernel<R, CPLX, MIXED, BM, BN>), dim3((unsigned)tiles), block, 0, s, g);
+      }
+      return hipGetLastError();
+    }
+  }
+  // Mixed domains: the real operand's plane and half the MFMAs left out; with
+  // every storage type native (dom_buf), both gathers take buffer loads.
+  if constexpr (MIXED && CPLX) {
+    if (g.dom == 1 || g.dom == 2) {
+      if (g.dom == 1 && g.dom_buf)
+        launch_dom<R, BM, BN, false, 1>(g, grid, s);
+      else if (g.dom == 1)
+        launch_dom<R, BM, BN, true, 1>(g, grid, s);
+      else if (g.dom_buf)
+        launch_dom<R, BM, BN, false, 2>(g, grid, s);
+      else
+        launch_dom<R, BM, BN, true, 2>(g, grid, s);
+      if (g.k_splits > 1) {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if (g.dom_buf)
+          hipLaunchKernelGGL((splitk_reduce_kernel<R, CPLX, false, BM, BN>), dim3((unsigned)tiles), block, 0, s, g);
+        else
+          hipLaunchKernelGGL((splitk_reduce_kernel<R, CPLX, true, BM, BN>), dim3((unsigned)tiles), block, 0, s, g);
       }
       return hipGetLastError();
     }

@@ -62,6 +62,13 @@ struct GemmArgs {
   // Every row pair (K pair) the gathers read in twos is adjacent in memory:
   // the tiles on the two-stage pipeline load two elements per lane.
   int vec = 0;
+  // Mixed domains (complex compute type, mixed storage types): 1 = A complex
+  // and B real, 2 = A real and B complex; 0 otherwise.
+  int dom = 0;
+  // dom != 0 with the complex operand and C in the compute type and the real
+  // operand in its real type, spans below 2 GiB: both gathers take buffer
+  // loads (a_lo / a_bytes and b_lo / b_bytes are set).
+  int dom_buf = 0;
 };
 
 // Row pitch, in elements, of an LDS tile stored [row][k]: the smallest pitch

@@ -196,6 +196,38 @@ def test_mixed_eltypes(oracle, TA, TB, TC, acc):
     run_and_check(oracle, Cv, Av, (1, 2), Bv, (2, 3), (1, 3), 1.5, 0.5, accumulator=acc, rtol=tol)
 
 
+@pytest.mark.parametrize("TA,TB,TC,acc,conjA,conjB,dom", [
+    (np.float64, np.complex128, np.complex128, None, False, True, 2),
+    (np.complex64, np.float64, np.complex128, None, True, False, 1),
+    (np.complex128, np.float32, np.complex128, torch.float32, False, False, 1),
+    (np.float32, np.complex64, np.complex64, None, False, True, 2),
+    (np.float64, np.float64, np.complex128, None, False, False, 0),
+])
+def test_mixed_domain_kernels(oracle, TA, TB, TC, acc, conjA, conjB, dom):
+    """ComplexRealKernel / RealComplexKernel (kernels.jl:125-160; cases after
+    test/execution/test_mixed.jl:28-69): one real operand keeps one plane and
+    two MFMAs per block; scattered strides, conj on the complex side, complex
+    beta.  Two real operands into a complex C take the full complex path."""
+    case(f"mixed domain {np.dtype(TA).name} x {np.dtype(TB).name} -> {np.dtype(TC).name} acc={acc}")
+    rng = np.random.default_rng(95)
+    d = 8
+    A = randn(rng, TA, 2 * d ** 4)
+    B = randn(rng, TB, d ** 4)
+    Cst = randn(rng, TC, d ** 4)
+    # C[a,b,c,d] = A[a,e,b,f] * B[f,c,e,d] with A a stride-2 slice along b
+    Av = dev_view(A, (d,) * 4, (1, d, 2 * d * d, 2 * d ** 3))
+    Bv = dev_view(B, (d,) * 4, (1, d, d * d, d ** 3))
+    Cv = dev_view(Cst, (d,) * 4, (1, d, d * d, d ** 3))
+    T = qs.compute_type(TORCH_OF[np.dtype(TA)], TORCH_OF[np.dtype(TB)], TORCH_OF[np.dtype(TC)], acc)
+    tol = max(RTOL[T], 1e-6 if np.dtype(TC) in (np.dtype(np.float32), np.dtype(np.complex64)) else 0)
+    beta = 0.3 - 0.7j
+    plan = run_and_check(oracle, Cv, Av, (1, 2, 3, 4), Bv, (4, 5, 2, 6), (1, 3, 5, 6), 1.5, beta,
+                         conjA=conjA, conjB=conjB, accumulator=acc, rtol=tol,
+                         path_modes=qs.PathModes("never", "never"))
+    want_dom = dom if not plan.swapped else {1: 2, 2: 1, 0: 0}[dom]
+    assert plan.info["mixed_domain"] == want_dom, (plan, plan.info)
+
+
 def test_one_rounding_into_narrow_c(oracle):
     # test/execution/test_c_panel.jl: 1, 2^-30, -1 along K survive only in the compute type
     case("one rounding: F64 x F64 -> F32, K = 3000")
