@@ -236,6 +236,11 @@ qs_status qs_add_plan_create(const qs_add_desc *desc, qs_add_plan **out);
 /* alpha and beta hold one value of the compute type (complex: 2 reals). */
 qs_status qs_add_execute(qs_add_plan *plan, const void *alpha, const void *beta, void *stream);
 qs_status qs_add_plan_destroy(qs_add_plan *plan);
+/* Point an add plan at other storages of the same layout (spans are checked
+ * at execute).  No reference counterpart (StridedNative plans nothing):
+ * repeated permutes and traces of one layout reuse the device tables. */
+qs_status qs_add_plan_rebind(qs_add_plan *plan, const void *a, int64_t a_base, int64_t a_len,
+                             void *c, int64_t c_base, int64_t c_len);
 
 /* ---- split precision ------------------------------------------------------
  * No counterpart in the reference.  A plan created with precision

@@ -124,6 +124,7 @@ SIGNATURES = {
     "qs_add_plan_create": (C.c_int, [C.POINTER(QSAddDesc), C.POINTER(C.c_void_p)]),
     "qs_add_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "qs_add_plan_destroy": (C.c_int, [C.c_void_p]),
+    "qs_add_plan_rebind": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64]),
     "qs_plan_set_error_report": (C.c_int, [C.c_void_p, C.c_int32]),
     "qs_plan_get_error_report": (C.c_int, [C.c_void_p, C.POINTER(QSErrorReport)]),
     "qs_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(QSTuneResult)]),

@@ -16,6 +16,7 @@ the sum of its two strides in A.
 from __future__ import annotations
 
 import ctypes as C
+from collections import OrderedDict
 
 import torch
 
@@ -89,18 +90,52 @@ def _run(Cv, Av, amap, pairs, conjA, alpha, beta):
     d.a, d.a_base, d.a_len = Av.parent.data_ptr(), Av.offset, Av.parent.numel()
     d.c, d.c_base, d.c_len = Cv.parent.data_ptr(), Cv.offset, Cv.parent.numel()
     lib = _lib.lib()
-    handle = C.c_void_p()
-    st = lib.qs_add_plan_create(C.byref(d), C.byref(handle))
+    handle = _cached_plan(d)
+    st = lib.qs_add_plan_rebind(handle, C.c_void_p(d.a), d.a_base, d.a_len, C.c_void_p(d.c), d.c_base, d.c_len)
     if st != 0:
         raise_status(st, _lib.last_error())
-    try:
-        a, b = _scalar_buffer(alpha, T), _scalar_buffer(beta, T)
-        s = C.c_void_p(torch.cuda.current_stream(Cv.parent.device).cuda_stream)
-        st = lib.qs_add_execute(handle, a, b, s)
-        if st != 0:
-            raise_status(st, _lib.last_error())
-    finally:
-        lib.qs_add_plan_destroy(handle)
+    a, b = _scalar_buffer(alpha, T), _scalar_buffer(beta, T)
+    s = C.c_void_p(torch.cuda.current_stream(Cv.parent.device).cuda_stream)
+    st = lib.qs_add_execute(handle, a, b, s)
+    if st != 0:
+        raise_status(st, _lib.last_error())
+
+
+# Add plans by layout: types, conj and the three groups (lengths and both
+# stride maps), with the device they live on.  A hit rebinds the storages and
+# reuses the device tables; the oldest plan is freed past CACHE_SIZE.
+CACHE_SIZE = 64
+_cache: "OrderedDict[tuple, C.c_void_p]" = OrderedDict()
+
+
+def _group_key(g):
+    n = g.ndim
+    return (n, tuple(g.lengths[:n]), tuple(g.strides[0][:n]), tuple(g.strides[1][:n]))
+
+
+def _cached_plan(d):
+    key = (torch.cuda.current_device(), d.a_type, d.c_type, d.compute_type, d.conj_a,
+           _group_key(d.m), _group_key(d.n), _group_key(d.k))
+    h = _cache.get(key)
+    if h is not None:
+        _cache.move_to_end(key)
+        return h
+    h = C.c_void_p()
+    st = _lib.lib().qs_add_plan_create(C.byref(d), C.byref(h))
+    if st != 0:
+        raise_status(st, _lib.last_error())
+    _cache[key] = h
+    while len(_cache) > CACHE_SIZE:
+        _, old = _cache.popitem(last=False)
+        _lib.lib().qs_add_plan_destroy(old)
+    return h
+
+
+def clear_plan_cache():
+    """Free every cached add plan."""
+    while _cache:
+        _, h = _cache.popitem(last=False)
+        _lib.lib().qs_add_plan_destroy(h)
 
 
 def tensoradd(C_, A_, pA, conjA, alpha, beta):

@@ -308,6 +308,15 @@ qs_status qs_add_execute(qs_add_plan *p, const void *alpha, const void *beta, vo
   return QS_OK;
 }
 
+qs_status qs_add_plan_rebind(qs_add_plan *p, const void *a, int64_t a_base, int64_t a_len, void *c, int64_t c_base,
+                             int64_t c_len) {
+  if (!p || !c || (!a && a_len > 0)) return fail(QS_ERR_ARGUMENT, "tensoradd!: null argument");
+  if (a_len < 0 || c_len < 0) return fail(QS_ERR_ARGUMENT, "tensoradd!: negative storage length");
+  p->desc.a = a; p->desc.a_base = a_base; p->desc.a_len = a_len;
+  p->desc.c = c; p->desc.c_base = c_base; p->desc.c_len = c_len;
+  return QS_OK;
+}
+
 qs_status qs_add_plan_destroy(qs_add_plan *p) {
   if (!p) return QS_OK;
   if (p->d_tables) (void)hipFree(p->d_tables);

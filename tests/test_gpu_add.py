@@ -144,3 +144,31 @@ def test_backend_tensoradd_and_tensortrace_torch():
     be.tensortrace(v, T, ((2,), ()), ((1,), (3,)), False, 1.0, 0.0)
     want = torch.einsum("iji->j", T.cpu())
     assert rel_fro(v.cpu().numpy(), want.numpy()) <= 1e-13
+
+
+def test_plan_cache_reuses_layouts_and_rebinds():
+    case("tensoradd plan cache: one plan per layout, storages rebound; a short storage still raises")
+    add = qs.add
+    add.clear_plan_cache()
+    rng = np.random.default_rng(12)
+    for i in range(3):   # same layout, new storages every time
+        a = rng.standard_normal((40, 24, 16))
+        c0 = rng.standard_normal((16, 40, 24))
+        Cv = dense(c0, DEV)
+        qs.tensoradd(Cv, dense(a, DEV), ((3, 1, 2), ()), False, 1.0, -0.5)
+        assert rel_fro(values(Cv), -0.5 * c0 + np.transpose(a, (2, 0, 1))) <= 1e-13
+        assert len(add._cache) == 1
+    t = rng.standard_normal((6, 9, 6))
+    v = dense(np.zeros(9), DEV)
+    qs.tensortrace(v, dense(t, DEV), ((2,), ()), ((1,), (3,)), False, 1.0, 0.0)
+    assert rel_fro(values(v), np.einsum("iji->j", t)) <= 1e-13
+    assert len(add._cache) == 2
+    # a cached plan rebound to a storage one element short raises before writing
+    a = rng.standard_normal((40, 24, 16))
+    cparent = rng.standard_normal(16 * 40 * 24 - 1)
+    Cv = SV(torch.from_numpy(cparent.copy()).to(DEV), (16, 40, 24), (1, 16, 16 * 40), 0)
+    with pytest.raises(qs.BoundsError):
+        qs.tensoradd(Cv, dense(a, DEV), ((3, 1, 2), ()), False, 1.0, 0.0)
+    assert np.array_equal(Cv.parent.cpu().numpy(), cparent)
+    add.clear_plan_cache()
+    assert len(add._cache) == 0
