@@ -44,6 +44,11 @@ struct Request {
 // In-flight batches, one stream each: a new batch can start while up to
 // three others run, so small requests do not queue behind two large ones.
 static constexpr int kSlots = 4;
+// Grouped mode: a request of at least this much work runs in a batch of its
+// own (a batch retires when its last member does, so small members would wait
+// for it), and large batches hold at most kSlots - 1 streams; small requests
+// may pass a large one that waits for a stream.
+static constexpr int64_t kLargeFlops = int64_t(4) << 30;
 
 }  // namespace qs
 
@@ -63,6 +68,7 @@ struct qs_server {
   hipEvent_t events[qs::kSlots] = {};
   qs::BatchBuffers bufs[qs::kSlots];
   bool busy[qs::kSlots] = {};
+  bool large[qs::kSlots] = {};   // the slot's batch is one large request
   std::vector<uint64_t> slot_ids[qs::kSlots];
   std::thread worker;
   bool stop = false;
@@ -92,6 +98,7 @@ static void retire(qs_server *s, int slot) {
   }
   s->slot_ids[slot].clear();
   s->busy[slot] = false;
+  s->large[slot] = false;
 }
 
 static void worker_loop(qs_server *s) {
@@ -111,7 +118,8 @@ static void worker_loop(qs_server *s) {
     for (int i = 0; i < s->slots; ++i)
       if (!s->busy[i]) { slot = i; break; }
     std::vector<uint64_t> batch;
-    if (slot >= 0) {
+    bool batch_large = false;
+    if (slot >= 0 && !s->grouped) {
       while (!s->pending.empty() && (int)batch.size() < s->max_batch) {
         Request &r = s->reqs[s->pending.front()];
         if (s->in_flight_bytes + r.bytes > s->budget) break;   // first-come first-served
@@ -119,6 +127,27 @@ static void worker_loop(qs_server *s) {
         s->peak_bytes = std::max(s->peak_bytes, s->in_flight_bytes);
         batch.push_back(s->pending.front());
         s->pending.pop_front();
+      }
+    } else if (slot >= 0) {
+      int large_busy = 0;
+      for (int i = 0; i < s->slots; ++i) large_busy += s->busy[i] && s->large[i];
+      for (auto it = s->pending.begin(); it != s->pending.end() && (int)batch.size() < s->max_batch;) {
+        Request &r = s->reqs[*it];
+        const bool big = r.flops >= kLargeFlops;
+        if (big && !batch.empty()) break;   // never shares a batch; the batch so far starts now
+        if (big && large_busy >= s->slots - 1) {   // waits for a stream; smaller requests behind it may pass
+          ++it;
+          continue;
+        }
+        if (s->in_flight_bytes + r.bytes > s->budget) break;   // first-come first-served within the budget
+        s->in_flight_bytes += r.bytes;
+        s->peak_bytes = std::max(s->peak_bytes, s->in_flight_bytes);
+        batch.push_back(*it);
+        it = s->pending.erase(it);
+        if (big) {
+          batch_large = true;
+          break;
+        }
       }
     }
     if (!batch.empty()) {
@@ -138,6 +167,7 @@ static void worker_loop(qs_server *s) {
       }
       s->slot_ids[slot] = batch;
       s->busy[slot] = true;
+      s->large[slot] = batch_large;
       s->batches++;
       s->batched += (int64_t)batch.size();
       lk.unlock();
