@@ -78,7 +78,7 @@ class QuasiStridedBackend:
         indA, indB, indC = contraction_labels(pA, pB, pAB)
         checkpoint = allocator.checkpoint() if callable(getattr(allocator, "checkpoint", None)) else None
         plan = planned(Cv, Av, indA, Bv, indB, indC, conjA=conjA, conjB=conjB, accumulator=self.accumulator,
-                       allocator=allocator)
+                       allocator=allocator, cache=True)
         try:
             execute(plan, alpha, beta)
         finally:

@@ -1,0 +1,155 @@
+"""Device-resident tensor networks: `ncon` over this engine.
+
+TensorOperations' `ncon(tensors, network; backend)` reaches the reference
+through `tensorcontract!` one pairwise contraction at a time
+(test/integrations/test_tensoroperations.jl:132-140), with the permutes and
+traces forwarded to StridedNative (src/integrations/tensoroperations.jl:90-107).
+Here the whole network runs on the device: partial traces through
+`tensortrace`, pairwise contractions through cached device plans, the final
+permutation through `tensoradd`, every intermediate a new tensor in HBM.
+
+Network convention (ncon's): `network[i]` labels the axes of `tensors[i]`;
+a positive label appears exactly twice (contracted; twice on one tensor is a
+partial trace), a negative label once (open).  The output axes are the open
+labels in the order -1, -2, ... unless `output` says otherwise; positive
+labels are contracted in increasing order unless `order` says otherwise.
+When the next label's two tensors are chosen, every label they share is
+contracted at once; tensors left with no shared label are combined by outer
+products, first two first.
+"""
+from __future__ import annotations
+
+from collections import Counter
+
+import torch
+
+from .add import tensoradd, tensortrace
+from .errors import ArgumentError, DimensionMismatch
+from .plan import execute, planned
+
+
+def ncon_steps(network, order=None, output=None):
+    """The host side of `ncon`: validate the network and return
+    `(traces, steps, output)`.
+
+    * `traces[i]` = (kept labels, traced label pairs) for a tensor that traces
+      a label with itself, else absent.
+    * `steps` = list of `(a, b, shared, labels)`: contract nodes `a` and `b`
+      over the labels `shared` into a new node (appended, numbered after the
+      inputs) whose axes carry `labels`.
+    * `output` = the final label order.
+    """
+    network = [tuple(int(l) for l in ix) for ix in network]
+    counts = Counter(l for ix in network for l in ix)
+    if any(l == 0 for l in counts):
+        raise ArgumentError("ncon: label 0 is not allowed")
+    for l, c in counts.items():
+        if l > 0 and c != 2:
+            raise ArgumentError(f"ncon: contracted label {l} appears {c} times (must be 2)")
+        if l < 0 and c != 1:
+            raise ArgumentError(f"ncon: open label {l} appears {c} times (must be 1)")
+    open_labels = sorted((l for l in counts if l < 0), reverse=True)
+    if output is None:
+        output = tuple(open_labels)
+    else:
+        output = tuple(int(l) for l in output)
+        if sorted(output) != sorted(open_labels):
+            raise ArgumentError(f"ncon: output {output} is not a permutation of the open labels {tuple(open_labels)}")
+    positive = sorted(l for l in counts if l > 0)
+    if order is None:
+        order = positive
+    else:
+        order = [int(l) for l in order]
+        if sorted(order) != positive:
+            raise ArgumentError(f"ncon: order {tuple(order)} is not a permutation of the contracted labels")
+
+    traces = {}
+    nodes = []
+    for i, ix in enumerate(network):
+        c = Counter(ix)
+        twice = [l for l in ix if c[l] == 2]
+        if twice:
+            pairs = sorted(set(twice), key=ix.index)
+            kept = tuple(l for l in ix if c[l] == 1)
+            traces[i] = (kept, tuple(pairs))
+            nodes.append(kept)
+        else:
+            nodes.append(ix)
+    done = {l for pair in traces.values() for l in pair[1]}
+    alive = list(range(len(nodes)))
+    steps = []
+    for l in order:
+        if l in done:
+            continue
+        holders = [n for n in alive if l in nodes[n]]
+        if len(holders) != 2:   # cannot happen after the count check and the traces
+            raise ArgumentError(f"ncon: label {l} is not on two tensors")
+        a, b = holders
+        shared = tuple(x for x in nodes[a] if x > 0 and x in nodes[b])
+        labels = tuple(x for x in nodes[a] if x not in shared) + tuple(x for x in nodes[b] if x not in shared)
+        steps.append((a, b, shared, labels))
+        nodes.append(labels)
+        alive = [n for n in alive if n not in (a, b)] + [len(nodes) - 1]
+        done.update(shared)
+    while len(alive) > 1:   # no shared labels left: outer products
+        a, b = alive[0], alive[1]
+        labels = nodes[a] + nodes[b]
+        steps.append((a, b, (), labels))
+        nodes.append(labels)
+        alive = alive[2:] + [len(nodes) - 1]
+    return traces, steps, output
+
+
+def ncon(tensors, network, conjlist=None, order=None, output=None):
+    """`ncon` on the device: returns a new tensor with the open labels in
+    `output` order (-1, -2, ... by default).  `conjlist[i]`: use conj of
+    `tensors[i]`."""
+    tensors = list(tensors)
+    network = [tuple(ix) for ix in network]
+    if len(tensors) != len(network):
+        raise ArgumentError(f"ncon: {len(tensors)} tensors but {len(network)} index lists")
+    conjlist = [False] * len(tensors) if conjlist is None else [bool(c) for c in conjlist]
+    if len(conjlist) != len(tensors):
+        raise ArgumentError("ncon: conjlist has the wrong length")
+    sizes = {}
+    for t, ix in zip(tensors, network):
+        if t.dim() != len(ix):
+            raise ArgumentError(f"ncon: a tensor of {t.dim()} axes has {len(ix)} labels")
+        if t.device.type != "cuda":
+            raise ArgumentError("ncon: the device engine takes tensors in GPU memory")
+        for n, l in zip(t.shape, ix):
+            if sizes.setdefault(int(l), int(n)) != int(n):
+                raise DimensionMismatch(f"ncon: label {l} has lengths {sizes[int(l)]} and {int(n)}")
+    traces, steps, output = ncon_steps(network, order, output)
+
+    nodes = []   # (tensor, labels, conj pending)
+    for i, (t, ix) in enumerate(zip(tensors, network)):
+        ix = tuple(int(l) for l in ix)
+        if i in traces:
+            kept, pairs = traces[i]
+            out = torch.empty([sizes[l] for l in kept], dtype=t.dtype, device=t.device)
+            p = tuple(ix.index(l) + 1 for l in kept)
+            q1 = tuple(ix.index(l) + 1 for l in pairs)
+            q2 = tuple(len(ix) - ix[::-1].index(l) for l in pairs)
+            tensortrace(out, t, (p, ()), (q1, q2), conjlist[i], 1.0, 0.0)
+            nodes.append((out, kept, False))
+        else:
+            nodes.append((t, ix, conjlist[i]))
+    for a, b, shared, labels in steps:
+        A, la, ca = nodes[a]
+        B, lb, cb = nodes[b]
+        dt = torch.promote_types(A.dtype, B.dtype)
+        out = torch.empty([sizes[l] for l in labels], dtype=dt, device=A.device)
+        plan = planned(out, A, la, B, lb, labels, conjA=ca, conjB=cb, cache=True)
+        try:
+            execute(plan, 1.0, 0.0)
+        finally:
+            plan.release()
+        nodes.append((out, labels, False))
+    X, lx, cx = nodes[-1]
+    if lx == output and not cx:
+        return X
+    res = torch.empty([sizes[l] for l in output], dtype=X.dtype, device=X.device)
+    perm = tuple(lx.index(l) + 1 for l in output)
+    tensoradd(res, X, (perm, ()), cx, 1.0, 0.0)
+    return res

@@ -10,6 +10,7 @@ storages and folded flags then go through the C ABI (`qs_plan_create`), and
 from __future__ import annotations
 
 import ctypes as C
+from collections import OrderedDict
 from dataclasses import dataclass
 
 import torch
@@ -122,6 +123,7 @@ class ContractPlan:
         self.swapped = swapped
         self._views = views  # keeps every operand alive while the plan lives
         self._al_keep = None  # the allocator bridge, kept until the plan is destroyed
+        self._cached = False  # owned by the layout cache
         info = _lib.QSPlanInfo()
         st = _lib.lib().qs_plan_get_info(handle, C.byref(info))
         if st != 0:
@@ -139,10 +141,11 @@ class ContractPlan:
         return self._handle
 
     def release(self):
-        """Free the plan (its workspace goes back to its allocator first)."""
-        if self._handle is not None:
+        """Free the plan (its workspace goes back to its allocator first).  A
+        plan from the layout cache stays cached; this drops the reference."""
+        if self._handle is not None and not self._cached:
             _lib.lib().qs_plan_destroy(self._handle)
-            self._handle = None
+        self._handle = None
         self._al_keep = None
 
     def release_workspace(self):
@@ -234,8 +237,10 @@ def _fill_group(g: AxisGroup) -> _lib.QSGroup:
 
 def planned(C_, A_, indA, B_, indB, indC, *, kernel=None, conjA=False, conjB=False, m_block=None,
             k_block=None, n_block=None, accumulator=None, path_modes=None, precision="plain",
-            target_error=0.0, allocator=None) -> ContractPlan:
-    """`planned` (plan.jl:197-262) up to the device plan."""
+            target_error=0.0, allocator=None, cache=False) -> ContractPlan:
+    """`planned` (plan.jl:197-262) up to the device plan.  `cache`: take the
+    device plan from the layout cache (device memory, the library's
+    allocator), rebinding its storages; its `release` leaves it cached."""
     Cv, Av, Bv = StridedView.of(C_), StridedView.of(A_), StridedView.of(B_)
     indA, indB, indC = tuple(indA), tuple(indB), tuple(indC)
     for name, ind, v in (("indA", indA, Av), ("indB", indB, Bv), ("indC", indC, Cv)):
@@ -294,6 +299,13 @@ def planned(C_, A_, indA, B_, indB, indC, *, kernel=None, conjA=False, conjB=Fal
     d.tile = int(getattr(kern, "tile", -1))
     d.k_splits = int(getattr(kern, "k_splits", 0))
     d.path_mode = 1 if (modes.dot == "never" and modes.outer == "never") else 0
+    views = (Av, Bv, Cv)
+    if cache and memory == _lib.QS_MEM_DEVICE and _alloc.is_default(allocator):
+        handle = _cached_handle(d)
+        plan = ContractPlan(handle, kern, mgroup, ngroup, kgroup, Av.parent, Av.offset, Bv.parent, Bv.offset,
+                            Cv.parent, Cv.offset, atransform, btransform, T, swapped, views)
+        plan._cached = True
+        return plan
     handle = C.c_void_p()
     keep = None
     if _alloc.is_default(allocator):
@@ -307,6 +319,51 @@ def planned(C_, A_, indA, B_, indB, indC, *, kernel=None, conjA=False, conjB=Fal
                         Cv.parent, Cv.offset, atransform, btransform, T, swapped, (Av, Bv, Cv))
     plan._al_keep = keep
     return plan
+
+
+# Device plans by layout for the TensorOperations backend, whose every call
+# plans afresh: the descriptor without its storages (types, conj, precision,
+# path and tile requests, the three groups with both stride maps) on one
+# device.  A hit rebinds the storages (qs_plan_rebind, which re-checks the
+# spans) and reuses the device tables and workspace: no device allocation per
+# call.  The oldest plan is freed past PLAN_CACHE_SIZE.
+PLAN_CACHE_SIZE = 64
+_plan_cache: "OrderedDict[tuple, C.c_void_p]" = OrderedDict()
+
+
+def _gkey(g):
+    n = g.ndim
+    return (n, tuple(g.lengths[:n]), tuple(g.strides[0][:n]), tuple(g.strides[1][:n]))
+
+
+def _cached_handle(d) -> C.c_void_p:
+    key = (torch.cuda.current_device(), d.a_type, d.b_type, d.c_type, d.compute_type, d.conj_a, d.conj_b,
+           d.precision, d.target_rel_err, d.tile, d.k_splits, d.path_mode, _gkey(d.m), _gkey(d.n), _gkey(d.k))
+    lib = _lib.lib()
+    h = _plan_cache.get(key)
+    if h is not None:
+        _plan_cache.move_to_end(key)
+        st = lib.qs_plan_rebind(h, C.c_void_p(d.a), d.a_base, d.a_len, C.c_void_p(d.b), d.b_base, d.b_len,
+                                C.c_void_p(d.c), d.c_base, d.c_len)
+        if st != 0:
+            raise_status(st, _lib.last_error())
+        return h
+    h = C.c_void_p()
+    st = lib.qs_plan_create(C.byref(d), C.byref(h))
+    if st != 0:
+        raise_status(st, _lib.last_error())
+    _plan_cache[key] = h
+    while len(_plan_cache) > PLAN_CACHE_SIZE:
+        _, old = _plan_cache.popitem(last=False)
+        lib.qs_plan_destroy(old)
+    return h
+
+
+def clear_plan_cache():
+    """Free every cached contraction plan."""
+    while _plan_cache:
+        _, h = _plan_cache.popitem(last=False)
+        _lib.lib().qs_plan_destroy(h)
 
 
 def plan_contract(C_, A_, indA, B_, indB, indC, *, kernel=None, conjA=False, conjB=False, m_block=None,

@@ -574,3 +574,45 @@ def test_autotune_keeps_results_and_c(oracle):
     assert (p2.info["tile_m"], p2.info["tile_n"], p2.info["k_splits"]) == \
         (plan.info["tile_m"], plan.info["tile_n"], plan.info["k_splits"])
     qs.autotune_clear()
+
+
+def test_backend_plan_cache_rebinds_layouts():
+    """The TensorOperations backend plans every call; the layout cache keeps
+    one device plan per layout and rebinds storages (no device allocation per
+    call), a non-default allocator bypasses it, and a short storage is still
+    rejected before any write."""
+    case("backend plan cache: repeated layouts, new storages")
+    pl = qs.plan
+    pl.clear_plan_cache()
+    rng = np.random.default_rng(97)
+    be = qs.QuasiStridedBackend()
+    for i in range(3):
+        A = torch.from_numpy(rng.standard_normal((30, 20, 24))).to(DEV)
+        B = torch.from_numpy(rng.standard_normal((20, 16))).to(DEV)
+        Cd = torch.zeros(30, 24, 16, dtype=torch.float64, device=DEV)
+        be.tensorcontract(Cd, A, ((1, 3), (2,)), False, B, ((1,), (2,)), False, ((1, 2, 3), ()), 1.0, 0.0)
+        torch.cuda.synchronize()
+        want = np.einsum("akb,kn->abn", A.cpu().numpy(), B.cpu().numpy())
+        assert rel_fro(Cd.cpu().numpy(), want) <= 1e-13
+        assert len(pl._plan_cache) == 1
+    A2 = torch.from_numpy(rng.standard_normal((32, 40))).to(DEV)
+    B2 = torch.from_numpy(rng.standard_normal((40, 8))).to(DEV)
+    C2 = torch.zeros(32, 8, dtype=torch.float64, device=DEV)
+    be.tensorcontract(C2, A2, ((1,), (2,)), False, B2, ((1,), (2,)), False, ((1, 2), ()), 1.0, 0.0)
+    assert len(pl._plan_cache) == 2
+    al = qs.TorchAllocator()
+    be.tensorcontract(C2, A2, ((1,), (2,)), False, B2, ((1,), (2,)), False, ((1, 2), ()), 2.0, 0.0, al)
+    torch.cuda.synchronize()
+    assert len(pl._plan_cache) == 2 and al.live_bytes == 0
+    assert rel_fro(C2.cpu().numpy(), 2.0 * (A2 @ B2).cpu().numpy()) <= 1e-13
+    # the cached plan of the first layout, rebound to a C storage one element short
+    A = torch.from_numpy(rng.standard_normal((30, 20, 24))).to(DEV)
+    B = torch.from_numpy(rng.standard_normal((20, 16))).to(DEV)
+    cpar = rng.standard_normal(30 * 24 * 16 - 1)
+    Cv = SV(torch.from_numpy(cpar.copy()).to(DEV), (30, 24, 16), (24 * 16, 16, 1))   # the first layout
+    with pytest.raises(qs.BoundsError):
+        be.tensorcontract(Cv, A, ((1, 3), (2,)), False, B, ((1,), (2,)), False, ((1, 2, 3), ()), 1.0, 0.0)
+    assert np.array_equal(Cv.parent.cpu().numpy(), cpar)
+    assert len(pl._plan_cache) == 2
+    pl.clear_plan_cache()
+    assert len(pl._plan_cache) == 0
