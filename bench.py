@@ -26,6 +26,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PEAK_FP64 = 78.6e12
+PEAK_FP32 = 157.3e12
 PEAK_HBM = 8.0e12
 
 
@@ -134,6 +135,33 @@ def bench_split(qs, torch, dev, d=64, reps=10):
                 "split_vs_plain_rel_fro": diff,
                 "int8_pops_equiv": round(flops * rep["moduli"] / (out["split_ms"] * 1e-3)
                                          / 1e15, 3)})
+    return out
+
+
+def bench_fp32(qs, torch, dev):
+    """Float32 on the FP32 MFMA (peak 157.3 TFLOP/s): the permuted all-32
+    workload and the rank-4 all-64 contraction, autotuned, launch period
+    from events at both ends of the timed region."""
+    out = {}
+    plan, flops, _, views, beta = config2(qs, torch, np.float32, dev)
+    qs.autotune(plan, reps=5)
+    _, period, _, _ = time_plan_region(qs, torch, plan, beta, 100, 10)
+    out["permuted_all32_f32"] = {"period_us": round(period * 1e6, 2), "tflops": round(flops / period / 1e12, 2),
+                                 "mfma_frac": round(flops / period / PEAK_FP32, 4), "plan": repr(plan)}
+    del plan, views
+    rng = np.random.default_rng(4)
+    d = 64
+    SV = qs.StridedView
+    mk = lambda: SV(torch.from_numpy(rng.standard_normal(d ** 4).astype(np.float32)).to(dev), (d,) * 4,
+                    colmajor((d,) * 4))
+    A, B = mk(), mk()
+    C = SV(torch.zeros(d ** 4, dtype=torch.float32, device=dev), (d,) * 4, colmajor((d,) * 4))
+    plan = qs.plan_contract(C, A, (1, 5, 2, 6), B, (5, 3, 6, 4), (1, 2, 3, 4))
+    qs.autotune(plan, reps=3)
+    _, period, _, _ = time_plan_region(qs, torch, plan, 0.0, 10, 3)
+    fl = 2 * 4096 ** 3
+    out["rank4_all64_f32"] = {"period_ms": round(period * 1e3, 4), "tflops": round(fl / period / 1e12, 2),
+                              "mfma_frac": round(fl / period / PEAK_FP32, 4), "plan": repr(plan)}
     return out
 
 
@@ -697,6 +725,7 @@ def main():
             extra[name] = {"gflops": round(fl * steps / w / 1e9, 1), "kernel_ms": round(km * 1e3, 4),
                            "mfma_frac": round(fl / km / PEAK_FP64, 4), "plan": repr(p)}
             del p, vw
+        extra["fp32"] = bench_fp32(qs, torch, dev)
         extra["split_rank4_all64_f64"] = bench_split(qs, torch, dev)
         extra["bandwidth_f64"] = bench_bandwidth(qs, torch, dev)
         extra["batched_f64"] = bench_batched(qs, torch, dev)
