@@ -360,7 +360,7 @@ def test_forced_split_k_ragged(oracle, tile):
     case(f"forced split-K 3, ragged K=1000, tile {tile}")
     rng = np.random.default_rng(40 + tile)
     M, K, N = 100, 1000, 70
-    for dtype in (np.float64, np.complex128):
+    for dtype in (np.float64, np.complex128, np.float32):
         Av = dev_view(randn(rng, dtype, M * K), (M, K), (1, M))
         Bv = dev_view(randn(rng, dtype, K * N), (K, N), (N, 1) if tile % 2 else (1, K))
         Cv = dev_view(randn(rng, dtype, M * N), (M, N), (1, M))
