@@ -616,3 +616,20 @@ def test_backend_plan_cache_rebinds_layouts():
     assert len(pl._plan_cache) == 2
     pl.clear_plan_cache()
     assert len(pl._plan_cache) == 0
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+def test_mixed_domain_split_k(oracle, tile):
+    """Mixed-domain kernels with forced split-K (partials of both planes, the
+    reduce kernel on the buffered and the 64-bit gather variants), ragged K."""
+    rng = np.random.default_rng(120 + tile)
+    M, K, N = 90, 700, 60
+    for TA, TB, TC in ((np.complex128, np.float64, np.complex128), (np.float64, np.complex128, np.complex128),
+                       (np.complex64, np.float64, np.complex128)):
+        case(f"mixed domain split-K 3, tile {tile}, {np.dtype(TA).name} x {np.dtype(TB).name}")
+        Av = dev_view(randn(rng, TA, M * K), (M, K), (1, M))
+        Bv = dev_view(randn(rng, TB, K * N), (K, N), (1, K))
+        Cv = dev_view(randn(rng, TC, M * N), (M, N), (1, M))
+        plan = run_and_check(oracle, Cv, Av, (1, 2), Bv, (2, 3), (1, 3), 0.75, 0.5 - 0.25j,
+                             kernel=qs.MFMAKernel(tile=tile, k_splits=3), path_modes=qs.PathModes("never", "never"))
+        assert plan.info["k_splits"] == 3 and plan.info["mixed_domain"] in (1, 2), (plan, plan.info)
