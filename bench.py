@@ -589,7 +589,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
+    # About 20 ms of warm-up on the headline: right after autotuning, the
+    # clocks take a few milliseconds to settle (a run timed then read 47 us
+    # per launch, later runs of the same plan 40-42 us).
+    ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune-cache", default=None,

@@ -626,7 +626,7 @@ struct TileGemm {
     // registers to spare: one element at a time.  Otherwise every old-C load
     // of a 16-row block row is issued before its first store.
     constexpr bool HEAVY = CPLX || MIXED || FM * FN * 4 * (int)sizeof(R) / 4 >= 128;
-    if (HEAVY || g.k_splits > 1 || g.beta_case == 0) {
+    if (g.k_splits > 1) {
 #pragma unroll
       for (int a = 0; a < FM; ++a)
 #pragma unroll
@@ -637,19 +637,45 @@ struct TileGemm {
           for (int b = 0; b < FN; ++b) {
             const int64_t col = n0 + wn * WTN + b * 16 + fr;
             if (col >= g.N) continue;
-            const R xr = acc[0][a][b][r];
-            const R xi = CPLX ? acc[P - 1][a][b][r] : (R)0;
-            if (g.k_splits == 1) {
-              store_c<R, CPLX, MIXED>(g, row, col, xr, xi);
-            } else {
-              R *part = (R *)g.partials;
-              const int64_t plane = g.M * g.N;
-              part[((int64_t)split * P) * plane + row * g.N + col] = xr;
-              if (CPLX) part[((int64_t)split * P + 1) * plane + row * g.N + col] = xi;
-            }
+            R *part = (R *)g.partials;
+            const int64_t plane = g.M * g.N;
+            part[((int64_t)split * P) * plane + row * g.N + col] = acc[0][a][b][r];
+            if (CPLX) part[((int64_t)split * P + 1) * plane + row * g.N + col] = acc[P - 1][a][b][r];
           }
         }
+    } else if (HEAVY) {
+      // A block row's C offsets read once; each element's old-C load and
+      // store one after the other (registers are short here).
+#pragma unroll
+      for (int a = 0; a < FM; ++a) {
+        int64_t moff[4], noff[FN];
+        bool rok[4], cok[FN];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = m0 + wm * WTM + a * 16 + mfma_row<R>(fk, r);
+          rok[r] = row < g.M;
+          moff[r] = rok[r] ? g.mC[row] : 0;
+        }
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          const int64_t col = n0 + wn * WTN + b * 16 + fr;
+          cok[b] = col < g.N;
+          noff[b] = cok[b] ? g.nC[col] : 0;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int b = 0; b < FN; ++b)
+            if (rok[r] && cok[b]) {
+              const int64_t idx = g.c_base + moff[r] + noff[b];
+              R o, oi;
+              load_c_old<R, CPLX, MIXED>(g, idx, o, oi);
+              store_c_at<R, CPLX, MIXED>(g, idx, acc[0][a][b][r], CPLX ? acc[P - 1][a][b][r] : (R)0, o, oi);
+            }
+      }
     } else {
+      // Every old-C load of a block row before its first store (none when
+      // beta == 0).
 #pragma unroll
       for (int a = 0; a < FM; ++a) {
         int64_t moff[4], noff[FN];
