@@ -419,7 +419,7 @@ struct TileGemm {
   __device__ static void run(const GemmArgs &g, int64_t tm, int64_t tn, int split, R *smem) {
     // Complex tiles up to 32 x 64 too: their gathers are small (ComplexF64
     // 32 x 64 x 16: 166 VGPRs, 139 -> 130 us at 1024^3).
-    constexpr bool DEEP = DEEPEN && !MIXED && (CPLX ? DOM == 0 && BM * BN <= 32 * 64 : BM * BN <= 64 * 128);
+    constexpr bool DEEP = DEEPEN && !MIXED && (CPLX ? BM * BN <= 32 * 64 : BM * BN <= 64 * 128);
     typedef typename Acc4<R>::type V;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
@@ -455,8 +455,8 @@ struct TileGemm {
       // Wave-uniform offsets (scalar loads) are fetched one stage ahead.
       auto deep = [&](auto ns) {
         constexpr int NS = decltype(ns)::value;
-        Gather<R, CPLX, MIXED, BM, BK, AKF, NS, VEC ? 2 : 1> ga;
-        Gather<R, CPLX, MIXED, BN, BK, BKF, NS, VEC ? 2 : 1> gb;
+        Gather<R, CA, MIXED, BM, BK, AKF, NS, VEC ? 2 : 1> ga;
+        Gather<R, CB, MIXED, BN, BK, BKF, NS, VEC ? 2 : 1> gb;
         ga.init(t, m0, g.M, g.a_base, g.mA, g.A, g.a_lo, g.a_bytes);
         gb.init(t, n0, g.N, g.b_base, g.nB, g.B, g.b_lo, g.b_bytes);
         typedef decltype(ga) GA;
@@ -475,25 +475,32 @@ struct TileGemm {
           R *Ws = smem + ((U + 1) & 1) * (SA + SB);
   #pragma unroll
           for (int ks = 0; ks < BK / 4; ++ks) {
-            R af[P][FM], bf[P][FN];
+            R af[PA][FM], bf[PB][FN];
   #pragma unroll
-            for (int p = 0; p < P; ++p) {
+            for (int p = 0; p < PA; ++p)
   #pragma unroll
               for (int a = 0; a < FM; ++a) af[p][a] = As[p * LA::plane + LA::at(wm * WTM + a * 16 + fr, ks * 4 + fk)];
   #pragma unroll
+            for (int p = 0; p < PB; ++p)
+  #pragma unroll
               for (int b = 0; b < FN; ++b) bf[p][b] = Bs[p * LB::plane + LB::at(wn * WTN + b * 16 + fr, ks * 4 + fk)];
-            }
   #pragma unroll
             for (int a = 0; a < FM; ++a)
   #pragma unroll
               for (int b = 0; b < FN; ++b) {
-                if (!CPLX) {
+                if constexpr (!CPLX) {
                   acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+                } else if constexpr (DOM == 1) {   // complex A, real B
+                  acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+                  acc[1][a][b] = mfma16(af[PA - 1][a], bf[0][b], acc[1][a][b]);
+                } else if constexpr (DOM == 2) {   // real A, complex B
+                  acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
+                  acc[1][a][b] = mfma16(af[0][a], bf[PB - 1][b], acc[1][a][b]);
                 } else {
                   acc[0][a][b] = mfma16(af[0][a], bf[0][b], acc[0][a][b]);
-                  acc[0][a][b] = mfma16(-af[P - 1][a], bf[P - 1][b], acc[0][a][b]);
-                  acc[P - 1][a][b] = mfma16(af[0][a], bf[P - 1][b], acc[P - 1][a][b]);
-                  acc[P - 1][a][b] = mfma16(af[P - 1][a], bf[0][b], acc[P - 1][a][b]);
+                  acc[0][a][b] = mfma16(-af[PA - 1][a], bf[PB - 1][b], acc[0][a][b]);
+                  acc[P - 1][a][b] = mfma16(af[0][a], bf[PB - 1][b], acc[P - 1][a][b]);
+                  acc[P - 1][a][b] = mfma16(af[PA - 1][a], bf[0][b], acc[P - 1][a][b]);
                 }
               }
           }
