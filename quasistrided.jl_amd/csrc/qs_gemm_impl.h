@@ -794,9 +794,9 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmArgs g) {
 // A batched launch (single storage type, no split-K): workgroup (tile,
 // blockIdx.z) runs the tile of batch blockIdx.z, whose offsets move the three
 // bases.
-template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF>
+template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool VEC = false>
 __global__ __launch_bounds__(256, 2) void gemm_batched_kernel(const GemmArgs g) {
-  typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, false> TG;
+  typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, false, VEC> TG;
   __shared__ R smem[TG::SMEM];
   const int64_t tiles_m = (g.M + BM - 1) / BM;
   const int64_t tiles_n = (g.N + BN - 1) / BN;
@@ -837,6 +837,23 @@ static hipError_t launch_tile_mixed(const GemmArgs &g, hipStream_t s) {
         GemmArgs part = g;
         part.batch_off = g.batch_off + 3 * z0;
         const dim3 bgrid((unsigned)tiles, 1, (unsigned)std::min<int64_t>(65535, g.batch - z0));
+        // Paired loads as in the one-contraction kernel: the batch offsets
+        // move the bases, not the tables, so the pairs stay adjacent.
+        if constexpr (!CPLX && BM * BN <= 64 * 128) {
+          if (g.vec) {
+            if (g.a_kfast && g.b_kfast)
+              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, true, true>), bgrid, dim3(256), 0, s, part);
+            else if (g.a_kfast)
+              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, false, true>), bgrid, dim3(256), 0, s, part);
+            else if (g.b_kfast)
+              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, false, true, true>), bgrid, dim3(256), 0, s, part);
+            else
+              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, false, false, true>), bgrid, dim3(256), 0, s, part);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            continue;
+          }
+        }
         if (g.a_kfast && g.b_kfast)
           hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, true>), bgrid, dim3(256), 0, s, part);
         else if (g.a_kfast)
