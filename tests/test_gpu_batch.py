@@ -66,6 +66,21 @@ def test_one_batch_label(dtype):
     assert bp.flops == (8 if cplx else 2) * i * j * k * b
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_paired_loads_on_every_tile(dtype):
+    case(f"batched paired loads on every tile: 48 x 40 x 66, b=5, {np.dtype(dtype).name}")
+    rng = np.random.default_rng(66)
+    i, j, k, b = 48, 40, 66, 5
+    Av = dev_view(randn(rng, dtype, i * k * b), (i, k, b), colmajor((i, k, b)))
+    Bv = dev_view(randn(rng, dtype, k * j * b), (k, j, b), colmajor((k, j, b)))
+    c0 = randn(rng, dtype, i * j * b)
+    for tile in range(5):
+        Cv = dev_view(c0, (i, j, b), colmajor((i, j, b)))
+        bp = run_batched(Cv, Av, (1, 2, 3), Bv, (2, 4, 3), (1, 4, 3), 1.25, 0.5, tile=tile)
+        assert bp.plan.info["paired_loads"] == 1, tile
+        bp.release()
+
+
 def test_two_batch_labels_permuted_conj():
     case("C[b2,j,i,b1] := conj(A[b1,i,k,b2]) * B[k,b2,j,b1], ComplexF64, strided parents")
     rng = np.random.default_rng(61)
