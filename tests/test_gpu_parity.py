@@ -633,3 +633,25 @@ def test_mixed_domain_split_k(oracle, tile):
         plan = run_and_check(oracle, Cv, Av, (1, 2), Bv, (2, 3), (1, 3), 0.75, 0.5 - 0.25j,
                              kernel=qs.MFMAKernel(tile=tile, k_splits=3), path_modes=qs.PathModes("never", "never"))
         assert plan.info["k_splits"] == 3 and plan.info["mixed_domain"] in (1, 2), (plan, plan.info)
+
+
+@pytest.mark.parametrize("tile", [1, 4])
+def test_mixed_domain_every_gather_direction(oracle, tile):
+    """Complex x real and real x complex on each of the four gather
+    directions (A K-fast or M-fast, B K-fast or N-fast): the two-stage
+    32 x 64 tile and the one-stage 64 x 64 tile, against the oracle."""
+    rng = np.random.default_rng(130 + tile)
+    M, K, N = 70, 90, 50
+    for TA, TB in ((np.complex128, np.float64), (np.float64, np.complex128)):
+        for akf in (False, True):
+            for bkf in (False, True):
+                case(f"mixed domain {np.dtype(TA).name} x {np.dtype(TB).name}, A K-fast {akf}, "
+                     f"B K-fast {bkf}, tile {tile}")
+                Av = dev_view(randn(rng, TA, M * K), (M, K), (K, 1) if akf else (1, M))
+                Bv = dev_view(randn(rng, TB, K * N), (K, N), (1, K) if bkf else (N, 1))
+                Cv = dev_view(randn(rng, np.complex128, M * N), (M, N), (1, M))
+                plan = run_and_check(oracle, Cv, Av, (1, 2), Bv, (2, 3), (1, 3), 1.25, 0.5 + 0.5j,
+                                     conjA=akf, conjB=bkf,
+                                     kernel=qs.MFMAKernel(tile=tile, k_splits=1),
+                                     path_modes=qs.PathModes("never", "never"))
+                assert plan.info["mixed_domain"] in (1, 2), (plan, plan.info)
