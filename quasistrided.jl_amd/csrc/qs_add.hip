@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void add_tile_kernel(const AddArgs a) {
 #pragma unroll
   for (int it = 0; it < 16; ++it)
     if (ok[it])
-      store_c_at<R, CPLX, MIXED>(g, idx[it], tile[0][lane][w + 4 * it], CPLX ? tile[P - 1][lane][w + 4 * it] : (R)0,
+      store_c_at<R, CPLX, MIXED, true>(g, idx[it], tile[0][lane][w + 4 * it], CPLX ? tile[P - 1][lane][w + 4 * it] : (R)0,
                                  o[it], oi[it]);
 }
 
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void add_line_kernel(const AddArgs a) {
   for (int u = 0; u < 4; ++u) load_c_old<R, CPLX, MIXED>(g, idx[u], o[u], oi[u]);
 #pragma unroll
   for (int u = 0; u < 4; ++u)
-    if (ok[u]) store_c_at<R, CPLX, MIXED>(g, idx[u], xr[u], xi[u], o[u], oi[u]);
+    if (ok[u]) store_c_at<R, CPLX, MIXED, true>(g, idx[u], xr[u], xi[u], o[u], oi[u]);
 }
 
 template <typename R, bool CPLX, bool MIXED>

@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void outer_kernel(const GemmArgs g) {
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int w = 0; w < W; ++w)
-    if (ok[w]) store_c_at<R, CPLX, MIXED>(g, idx[w], xr[w], xi[w], o[w], oi[w]);
+    if (ok[w]) store_c_at<R, CPLX, MIXED, true>(g, idx[w], xr[w], xi[w], o[w], oi[w]);
   }
 }
 

@@ -321,7 +321,7 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(CrtOperand o, const Cr
       word[q] = (int)(packed ^ 0x80808080u);            // two's complement of the residue
     }
     const qs_v4i out = {word[0], word[1], word[2], word[3]};
-    *(QS_GLOBAL qs_v4i *)(o.res + ((int64_t)l * RB * KB + frag) * 1024 + lane * 16) = out;
+    __builtin_nontemporal_store(out, (QS_GLOBAL qs_v4i *)(o.res + ((int64_t)l * RB * KB + frag) * 1024 + lane * 16));
   }
 }
 
@@ -571,8 +571,8 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       if (!ok[j]) continue;
-      if (g.compute_type == QS_F64) store_c_at<double, false, true>(g, idx[j], v[j], 0.0, o[j], 0.0);
-      else store_c_at<float, false, true>(g, idx[j], (float)v[j], 0.0f, (float)o[j], 0.0f);
+      if (g.compute_type == QS_F64) store_c_at<double, false, true, true>(g, idx[j], v[j], 0.0, o[j], 0.0);
+      else store_c_at<float, false, true, true>(g, idx[j], (float)v[j], 0.0f, (float)o[j], 0.0f);
     }
   }
 }
