@@ -71,7 +71,9 @@ __device__ __forceinline__ void dot_store(const GemmArgs &g, const DotArgs &d, i
 // lane sums its own K residue class (k = lane mod 64) of each of the 4 rows
 // in order, with 4 k-steps of loads issued together; the K offsets and the
 // vector elements of a step are loaded once for the 4 rows.  Then a fixed
-// butterfly reduction per row.
+// butterfly reduction per row.  The matrix is read with non-temporal loads:
+// it streams through once, and the vector and K table stay cached
+// (65536 x 2048 doubles: 217 -> 208 us).
 template <typename R, bool CPLX, bool MIXED>
 __global__ __launch_bounds__(256) void dot_klanes_kernel(const GemmArgs g, const DotArgs d) {
   constexpr int RW = 4, U = 4;
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(256) void dot_klanes_kernel(const GemmArgs g, const
       const int64_t kx = d.kX[kc];
       load_elem<R, CPLX, MIXED>(d.V, d.v_type, d.vbase + d.kv[kc], vr[u], vi[u]);
 #pragma unroll
-      for (int r = 0; r < RW; ++r) load_elem<R, CPLX, MIXED>(d.X, d.x_type, xrow[r] + kx, xr[u][r], xi[u][r]);
+      for (int r = 0; r < RW; ++r) load_elem<R, CPLX, MIXED, true>(d.X, d.x_type, xrow[r] + kx, xr[u][r], xi[u][r]);
     }
     __builtin_amdgcn_sched_barrier(0);   // every load of the step is in flight
 #pragma unroll

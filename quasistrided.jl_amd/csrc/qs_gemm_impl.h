@@ -64,11 +64,21 @@ __device__ __forceinline__ float qs_fma(float a, float b, float c) { return __bu
 // Element `idx` of storage `p` as the compute type.  MIXED: the storage type
 // `t` is a runtime value (uniform across the workgroup); otherwise it is the
 // compute type itself.
-template <typename R, bool CPLX, bool MIXED>
+// NT: a non-temporal load, for operands streamed once (a dot product's
+// matrix operand, far larger than the caches).
+template <typename R, bool CPLX, bool MIXED, bool NT = false>
 __device__ __forceinline__ void load_elem(const void *p, int t, int64_t idx, R &re, R &im) {
   if (!MIXED) {
     const QS_GLOBAL R *q = (const QS_GLOBAL R *)p;
-    if (!CPLX) {
+    if constexpr (NT) {
+      if (!CPLX) {
+        re = __builtin_nontemporal_load(q + idx);
+        im = 0;
+      } else {
+        re = __builtin_nontemporal_load(q + 2 * idx);
+        im = __builtin_nontemporal_load(q + 2 * idx + 1);
+      }
+    } else if (!CPLX) {
       re = q[idx];
       im = 0;
     } else {
