@@ -12,7 +12,8 @@ Network convention (ncon's): `network[i]` labels the axes of `tensors[i]`;
 a positive label appears exactly twice (contracted; twice on one tensor is a
 partial trace), a negative label once (open).  The output axes are the open
 labels in the order -1, -2, ... unless `output` says otherwise; positive
-labels are contracted in increasing order unless `order` says otherwise.
+labels are contracted in increasing order unless `order` says otherwise
+(`order="optimal"`: the least-multiplication order, `optimal_order`).
 When the next label's two tensors are chosen, every label they share is
 contracted at once; tensors left with no shared label are combined by outer
 products, first two first.
@@ -100,10 +101,117 @@ def ncon_steps(network, order=None, output=None):
     return traces, steps, output
 
 
+def _free_labels(ix):
+    """The labels of one tensor after its partial traces."""
+    c = Counter(ix)
+    return frozenset(l for l in ix if c[l] == 1)
+
+
+def contraction_cost(network, sizes, order=None):
+    """Multiplications of the pairwise sequence `ncon_steps(network, order)`
+    runs: each step costs the product of the lengths of every label on its
+    two operands (shared ones once).  Partial traces are not counted: every
+    order performs the same ones."""
+    _, steps, _ = ncon_steps(network, order)
+    nodes = [tuple(_free_labels(tuple(int(l) for l in ix))) for ix in network]
+    total = 0
+    for a, b, _shared, labels in steps:
+        cost = 1
+        for l in set(nodes[a]) | set(nodes[b]):
+            cost *= int(sizes[l])
+        total += cost
+        nodes.append(labels)
+    return total
+
+
+def optimal_order(network, sizes):
+    """A contraction order (a permutation of the positive labels, as `ncon`'s
+    `order`) of least `contraction_cost`.
+
+    Dynamic programming over subsets of tensors: the best pairwise tree of a
+    subset S is the cheapest split S = S1 + S2 into two parts that share a
+    label, each part contracted by its own best tree first; a step costs the
+    product of the lengths of the labels on its two operands.  Disconnected
+    components are searched apart (`ncon` joins them by outer products at the
+    end).  The tree is turned into an order by listing, children first, one
+    shared label per step: `ncon` contracts every label two nodes share when
+    it reaches one of them.  Exhaustive, so meant for networks of up to
+    about 12 tensors (3^n splits)."""
+    network = [tuple(int(l) for l in ix) for ix in network]
+    ncon_steps(network)   # the same label checks as ncon
+    n = len(network)
+    if n > 16:
+        raise ArgumentError(f"optimal_order: {n} tensors is beyond the exhaustive search (at most 16)")
+    free = [_free_labels(ix) for ix in network]
+    labels_of = {}
+
+    def labels(mask):   # open labels of the subnetwork `mask`
+        if mask not in labels_of:
+            c = Counter(l for i in range(n) if mask >> i & 1 for l in free[i])
+            labels_of[mask] = frozenset(l for l, k in c.items() if k == 1)
+        return labels_of[mask]
+
+    best = {1 << i: (0, None) for i in range(n)}
+
+    def solve(mask):
+        if mask in best:
+            return best[mask][0]
+        low = mask & -mask
+        top = None
+        sub = (mask - 1) & mask
+        while sub:
+            if sub & low:   # each split once: the part holding the lowest tensor
+                rest = mask ^ sub
+                if labels(sub) & labels(rest):
+                    step = 1
+                    for l in labels(sub) | labels(rest):
+                        step *= int(sizes[l])
+                    c1 = solve(sub)
+                    c2 = solve(rest)
+                    if c1 is not None and c2 is not None and (top is None or c1 + c2 + step < top[0]):
+                        top = (c1 + c2 + step, (sub, rest))
+            sub = (sub - 1) & mask
+        best[mask] = top if top is not None else (None, None)
+        return best[mask][0]
+
+    # connected components of the tensor graph
+    comps, seen = [], 0
+    for i in range(n):
+        if seen >> i & 1:
+            continue
+        comp, grow = 1 << i, True
+        while grow:
+            grow = False
+            for j in range(n):
+                if not comp >> j & 1 and labels(comp) & free[j]:
+                    comp |= 1 << j
+                    grow = True
+        seen |= comp
+        comps.append(comp)
+
+    order = []
+
+    def emit(mask):
+        split = best[mask][1]
+        if split is None:
+            return
+        s1, s2 = split
+        emit(s1)
+        emit(s2)
+        order.append(min(labels(s1) & labels(s2)))
+
+    for comp in comps:
+        solve(comp)
+        emit(comp)
+    positive = sorted({l for ix in network for l in ix if l > 0})
+    return order + [l for l in positive if l not in order]
+
+
 def ncon(tensors, network, conjlist=None, order=None, output=None):
     """`ncon` on the device: returns a new tensor with the open labels in
     `output` order (-1, -2, ... by default).  `conjlist[i]`: use conj of
-    `tensors[i]`."""
+    `tensors[i]`.  `order="optimal"` contracts in the order `optimal_order`
+    finds for these tensors' lengths."""
     tensors = list(tensors)
     network = [tuple(ix) for ix in network]
     if len(tensors) != len(network):
@@ -120,6 +228,10 @@ def ncon(tensors, network, conjlist=None, order=None, output=None):
         for n, l in zip(t.shape, ix):
             if sizes.setdefault(int(l), int(n)) != int(n):
                 raise DimensionMismatch(f"ncon: label {l} has lengths {sizes[int(l)]} and {int(n)}")
+    if isinstance(order, str):
+        if order != "optimal":
+            raise ArgumentError(f"ncon: order {order!r} (a label sequence or \"optimal\")")
+        order = optimal_order(network, sizes)
     traces, steps, output = ncon_steps(network, order, output)
 
     nodes = []   # (tensor, labels, conj pending)

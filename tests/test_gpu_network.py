@@ -74,3 +74,28 @@ def test_label_length_mismatch():
     case("ncon: a label with two lengths raises DimensionMismatch")
     with pytest.raises(qs.DimensionMismatch):
         qs.ncon([dev(np.zeros((3, 4))), dev(np.zeros((5, 2)))], [[1, -1], [1, -2]])
+
+
+def test_optimal_order_mps_transfer():
+    case("ncon order='optimal' on the MPS transfer network, F64, bonds 24/40, against einsum and the given order")
+    rng = np.random.default_rng(11)
+    D, d, w = 24, 3, 5
+    L = rng.standard_normal((D, w, D))
+    psi = rng.standard_normal((D, d, 40))
+    W = rng.standard_normal((w, w, d, d))
+    net = [[1, 2, 3], [3, 4, -1], [2, -2, 4, 5], [1, 5, -3]]
+    sizes = {1: D, 2: w, 3: D, 4: d, -1: 40, -2: w, 5: d, -3: 40}
+    order = qs.optimal_order(net, sizes)
+    assert qs.contraction_cost(net, sizes, order) <= qs.contraction_cost(net, sizes, [3, 2, 4, 1, 5])
+    out = qs.ncon([dev(L), dev(psi), dev(W), dev(psi)], net, order="optimal")
+    want = np.einsum("awx,xsb,wvst,atc->bvc", L, psi, W, psi)
+    assert rel_fro(out.cpu().numpy(), want) <= 1e-13
+    # a chain where the search departs from increasing labels: (A B) C
+    A, B, C = rng.standard_normal((10, 100)), rng.standard_normal((100, 5)), rng.standard_normal((5, 50))
+    chain = [[-1, 2], [2, 1], [1, -2]]
+    assert qs.optimal_order(chain, {-1: 10, 2: 100, 1: 5, -2: 50}) == [2, 1]
+    got = qs.ncon([dev(A), dev(B), dev(C)], chain, order="optimal")
+    assert rel_fro(got.cpu().numpy(), A @ B @ C) <= 1e-13
+    with pytest.raises(qs.ArgumentError, match="optimal"):
+        qs.ncon([dev(L), dev(psi), dev(W), dev(psi)], net, order="greedy")
+    qs.plan.clear_plan_cache()

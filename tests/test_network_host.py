@@ -49,3 +49,67 @@ def test_bad_order_and_output():
         steps_of([[1, -1], [1, -2]], order=[2])
     with pytest.raises(qs.ArgumentError, match="output"):
         steps_of([[1, -1], [1, -2]], output=[-1, -3])
+
+
+# ---- contraction-order search (`optimal_order`, `contraction_cost`)
+
+def test_matrix_chain_known_answer():
+    # A[-1,2] 10x100, B[2,1] 100x5, C[1,-2] 5x50: increasing labels contract
+    # B C first (100*5*50 + 10*100*50 = 75000 multiplications); (A B) C costs
+    # 10*100*5 + 10*5*50 = 7500, the textbook matrix-chain optimum.
+    net = [[-1, 2], [2, 1], [1, -2]]
+    sizes = {-1: 10, 2: 100, 1: 5, -2: 50}
+    assert qs.contraction_cost(net, sizes) == 75000
+    assert qs.optimal_order(net, sizes) == [2, 1]
+    assert qs.contraction_cost(net, sizes, [2, 1]) == 7500
+
+
+def _brute_min(net, sizes):
+    import itertools
+    labels = sorted({l for ix in net for l in ix if l > 0})
+    return min(qs.contraction_cost(net, sizes, list(p)) for p in itertools.permutations(labels))
+
+
+def _random_network(rng, n_tensors, n_bonds, n_open):
+    import numpy as np
+    net = [[] for _ in range(n_tensors)]
+    for b in range(1, n_bonds + 1):   # a spanning chain first, so the network is connected
+        if b < n_tensors:
+            i, j = b - 1, b
+        else:
+            i, j = rng.choice(n_tensors, size=2, replace=False)
+        net[int(i)].append(b)
+        net[int(j)].append(b)
+    for o in range(1, n_open + 1):
+        net[int(rng.integers(n_tensors))].append(-o)
+    for ix in net:
+        rng.shuffle(ix)
+    sizes = {l: int(rng.choice([2, 3, 5, 8, 16, 40])) for ix in net for l in ix}
+    return [list(map(int, ix)) for ix in net], sizes
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_optimal_order_matches_brute_force(seed):
+    # every order of the contracted labels (each a different pairwise tree or
+    # the same tree in another sequence) against the dynamic program
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n = 3 + seed % 3
+    net, sizes = _random_network(rng, n, n + 1 + seed % 2, 3)
+    order = qs.optimal_order(net, sizes)
+    assert sorted(order) == sorted({l for ix in net for l in ix if l > 0})
+    assert qs.contraction_cost(net, sizes, order) == _brute_min(net, sizes)
+
+
+def test_optimal_order_traces_and_components():
+    # a partial trace on tensor 0 and two components joined by an outer product
+    net = [[1, 1, -1, 2], [2, 3, -2], [3, -3], [4, -4], [4, 5], [5, -5]]
+    sizes = {1: 3, -1: 4, 2: 30, 3: 2, -2: 50, -3: 60, 4: 7, -4: 9, 5: 100, -5: 2}
+    order = qs.optimal_order(net, sizes)
+    assert sorted(order) == [1, 2, 3, 4, 5]
+    assert qs.contraction_cost(net, sizes, order) == _brute_min(net, sizes)
+
+
+def test_optimal_order_rejects_bad_networks():
+    with pytest.raises(qs.ArgumentError, match="contracted label 1 appears 1"):
+        qs.optimal_order([[1, -1], [2, -2]], {1: 2, 2: 2, -1: 2, -2: 2})
