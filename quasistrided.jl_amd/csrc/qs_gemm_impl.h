@@ -248,7 +248,10 @@ __device__ const double qs_zero16[2] = {0.0, 0.0};
 // -- two rows of one K step (KF == false) or two K steps of one row -- that
 // the host has found adjacent in memory for every such pair of the operand's
 // tables, in one 2 x ESZ-byte buffer load.  Half the load instructions.
-template <typename R, bool CPLX, bool MIXED, int ROWS, int BK, bool KF, int VS = 2, int VW = 1>
+//
+// AUX: the buffer loads' cache-policy bits (2: non-temporal, for an operand
+// that only one workgroup reads).
+template <typename R, bool CPLX, bool MIXED, int ROWS, int BK, bool KF, int VS = 2, int VW = 1, int AUX = 0>
 struct Gather {
   static constexpr int P = CPLX ? 2 : 1;
   static constexpr int NL = (ROWS * BK + 256 * VW - 1) / (256 * VW);   // loads per thread
@@ -335,22 +338,22 @@ struct Gather {
           struct Two { R x, y; };
           Two v;
           if constexpr (ESZ == 4)
-            v = __builtin_bit_cast(Two, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
+            v = __builtin_bit_cast(Two, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, AUX));
           else
-            v = __builtin_bit_cast(Two, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+            v = __builtin_bit_cast(Two, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, AUX));
           val[S][r][0] = v.x;
           val[S][r][1] = v.y;
         } else if constexpr (ESZ == 4) {
-          val[S][r][0] = __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0));
+          val[S][r][0] = __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, AUX));
         } else if constexpr (ESZ == 8 && !CPLX) {
-          val[S][r][0] = __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
+          val[S][r][0] = __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, AUX));
         } else if constexpr (ESZ == 8) {
-          const uint64_t u = __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
+          const uint64_t u = __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, AUX));
           val[S][r][0] = __builtin_bit_cast(R, (uint32_t)u);
           val[S][r][P - 1] = __builtin_bit_cast(R, (uint32_t)(u >> 32));
         } else {
           struct Pair { R re, im; };
-          const Pair v = __builtin_bit_cast(Pair, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+          const Pair v = __builtin_bit_cast(Pair, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, AUX));
           val[S][r][0] = v.re;
           val[S][r][P - 1] = v.im;
         }
@@ -414,7 +417,8 @@ __device__ __forceinline__ void sched_interleave() {
 // RealComplexKernel (src/microkernels/kernels.jl:125-160).  The real operand
 // keeps one LDS plane and a real gather, and each block takes two MFMAs
 // (re += ar br, im += ai br; or re += ar br, im += ar bi) instead of four.
-template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED, bool VEC = false, int DOM = 0>
+template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED, bool VEC = false, int DOM = 0,
+          int BAUX = 0>
 struct TileGemm {
   static constexpr int P = CPLX ? 2 : 1;
   static constexpr bool CA = CPLX && DOM != 2, CB = CPLX && DOM != 1;   // complex A / B
@@ -476,7 +480,7 @@ struct TileGemm {
       auto deep = [&](auto ns) {
         constexpr int NS = decltype(ns)::value;
         Gather<R, CA, MIXED, BM, BK, AKF, NS, VEC ? 2 : 1> ga;
-        Gather<R, CB, MIXED, BN, BK, BKF, NS, VEC ? 2 : 1> gb;
+        Gather<R, CB, MIXED, BN, BK, BKF, NS, VEC ? 2 : 1, BAUX> gb;
         ga.init(t, m0, g.M, g.a_base, g.mA, g.A, g.a_lo, g.a_bytes);
         gb.init(t, n0, g.N, g.b_base, g.nB, g.B, g.b_lo, g.b_bytes);
         typedef decltype(ga) GA;
@@ -799,9 +803,10 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const GemmArgs g) {
   }
 }
 
-template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED, bool VEC = false, int DOM = 0>
+template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED, bool VEC = false, int DOM = 0,
+          int BAUX = 0>
 __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmArgs g) {
-  typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, MIXED, VEC, DOM> TG;
+  typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, MIXED, VEC, DOM, BAUX> TG;
   __shared__ R smem[TG::SMEM];
   const int64_t tiles_m = (g.M + BM - 1) / BM;
   const int64_t tiles_n = (g.N + BN - 1) / BN;
@@ -900,7 +905,21 @@ static hipError_t launch_tile_mixed(const GemmArgs &g, hipStream_t s) {
   // Paired loads: real, one storage type, tiles on the two-stage pipeline.
   if constexpr (!MIXED && !CPLX && BM * BN <= 64 * 128) {
     if (g.vec) {
-      if (g.a_kfast && g.b_kfast)
+      // One row of 32-row tiles (M <= 32, B read in place): each B element
+      // is read by one workgroup, so B's loads are non-temporal.
+      const bool streamed = BM == 32 && g.M <= 32 && g.k_splits == 1;
+      if (streamed) {
+        if constexpr (BM == 32) {
+          if (g.a_kfast && g.b_kfast)
+            hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, true, true, false, true, 0, 2>), grid, block, 0, s, g);
+          else if (g.a_kfast)
+            hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, true, false, false, true, 0, 2>), grid, block, 0, s, g);
+          else if (g.b_kfast)
+            hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, false, true, false, true, 0, 2>), grid, block, 0, s, g);
+          else
+            hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, false, false, false, true, 0, 2>), grid, block, 0, s, g);
+        }
+      } else if (g.a_kfast && g.b_kfast)
         hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, true, true, false, true>), grid, block, 0, s, g);
       else if (g.a_kfast)
         hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, true, false, false, true>), grid, block, 0, s, g);

@@ -463,6 +463,33 @@ def test_paired_loads_match_single_loads(oracle, dtype, akf):
             assert np.array_equal(r, results[0])
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("akf", [False, True])
+def test_one_tile_row_streams_b(oracle, dtype, akf):
+    """M <= 32 on 32-row tiles: one tile row, so each B element is read by one
+    workgroup and B's loads are non-temporal.  The results match the oracle
+    and, bit for bit, the 128 x 128 tile's; ragged N and K, and an odd M (no
+    paired loads for an M-fast A) included."""
+    rng = np.random.default_rng(140 + int(akf))
+    K, N = 301, 517
+    for m in (16, 32, 31):
+        case(f"streamed B {np.dtype(dtype).name}, A {'K' if akf else 'M'}-fast, M={m}, K={K}, N={N}")
+        A = randn(rng, dtype, m * K)
+        B = randn(rng, dtype, K * N)
+        C0 = randn(rng, dtype, m * N)
+        results = []
+        for tile in (4, 0):
+            Av = dev_view(A, (m, K), (K, 1) if akf else (1, m))
+            Bv = dev_view(B, (K, N), (1, K))
+            Cv = dev_view(C0, (m, N), (1, m))
+            plan = run_and_check(oracle, Cv, Av, (1, 2), Bv, (2, 3), (1, 3), 1.5, 0.5,
+                                 kernel=qs.MFMAKernel(tile=tile, k_splits=1), path_modes=qs.PathModes("never", "never"))
+            assert plan.info["k_splits"] == 1 and plan.info["tile_m"] == (32 if tile == 4 else 128), plan.info
+            assert not plan.swapped and plan.info["m"] == m, plan
+            results.append(values(Cv).copy())
+        assert np.array_equal(results[0], results[1])
+
+
 def test_host_memory_operands(oracle):
     case("host-memory operands staged through the library")
     rng = np.random.default_rng(12)
