@@ -201,7 +201,10 @@ qs_status qs_execute(qs_plan *plan, const void *alpha, const void *beta, void *s
 qs_status qs_plan_get_info(const qs_plan *plan, qs_plan_info *out);
 
 /* Point an existing plan at other storages of the same layout (device
- * memory only); re-runs the span check.  Lets one plan serve many requests. */
+ * memory only).  The new spans are checked exactly by the next execute
+ * (BoundsError before any write; alpha == 0 never reads A or B, so their
+ * spans are checked only when alpha != 0).  Lets one plan serve many
+ * requests. */
 qs_status qs_plan_rebind(qs_plan *plan, const void *a, int64_t a_base, int64_t a_len,
                          const void *b, int64_t b_base, int64_t b_len,
                          void *c, int64_t c_base, int64_t c_len);
@@ -295,10 +298,13 @@ qs_status qs_autotune_clear(void);
  * (src/execution/execute.jl:11-20): tensor-network sweeps issue many small
  * contractions of uneven shape, and a group runs all of them in one
  * persistent launch per compute type and gather direction.  Members are
- * existing device-memory plans on the tiled GEMM path (they must outlive the
- * group; rebinding a member's storages requires rebuilding the group).
- * Each member computes all of K in registers, so its result does not depend
- * on the other members. */
+ * existing device-memory plans on the tiled GEMM path.  They must outlive the
+ * group and every graph captured from it.  A member rebound after the group
+ * was built (qs_plan_rebind) has the group's device tables rebuilt at the
+ * next execute or capture; a graph replays the storages it was captured
+ * with.  Every member's spans are checked exactly before an execute or a
+ * capture launches anything (QS_ERR_BOUNDS).  Each member computes all of K
+ * in registers, so its result does not depend on the other members. */
 typedef struct qs_group_plan qs_group_plan;
 typedef struct qs_graph qs_graph;
 
@@ -308,7 +314,8 @@ qs_status qs_group_create(qs_plan *const *plans, int32_t n, const int32_t *tiles
 
 /* A batched contraction as a group: n members that all run `plan`, member i
  * with a_offsets[i] / b_offsets[i] / c_offsets[i] (elements) added to the
- * plan's A / B / C bases.  This is the device form of batch labels (a label
+ * plan's A / B / C bases (every member's spans are checked at create:
+ * QS_ERR_BOUNDS).  This is the device form of batch labels (a label
  * in A, B and C), which the reference rejects (src/planning/labels.jl:48-54,
  * the check `plan_contract` keeps); it is an extension with no counterpart
  * there.  Members' C spans must not overlap.  `tile` as in qs_group_create. */
@@ -366,6 +373,22 @@ qs_status qs_server_drain(qs_server *s);
 qs_status qs_server_stats_get(qs_server *s, qs_server_stats *out);
 qs_status qs_server_reset_stats(qs_server *s);
 qs_status qs_server_destroy(qs_server *s);
+
+/* ---- allocation counts ----------------------------------------------------
+ * No counterpart in the reference, whose steady-state `execute!` allocates
+ * nothing (test/execution/test_execute.jl:72-94).  Every device and pinned
+ * host allocation the library makes, and every workspace block it takes from
+ * a caller's qs_allocator, is counted since the library was loaded; a test
+ * reads the counts around repeated executes of one plan and expects them to
+ * stay put. */
+typedef struct qs_alloc_stats {
+  int64_t device_allocs, device_frees;   /* hipMalloc / hipFree calls that succeeded */
+  int64_t device_bytes;                  /* bytes of those hipMallocs              */
+  int64_t pinned_allocs, pinned_frees;   /* hipHostMalloc / hipHostFree            */
+  int64_t bridge_allocs, bridge_frees;   /* qs_allocator alloc / free callbacks    */
+} qs_alloc_stats;
+
+qs_status qs_alloc_stats_get(qs_alloc_stats *out);
 
 /* Host-side offset enumeration, exactly `fill_offsets!`
  * (src/layout/axis_group.jl:87-145) for a two-map group: writes `count`

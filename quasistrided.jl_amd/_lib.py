@@ -99,6 +99,12 @@ class QSTargetProfile(C.Structure):
     ]
 
 
+class QSAllocStats(C.Structure):
+    _fields_ = [("device_allocs", C.c_int64), ("device_frees", C.c_int64), ("device_bytes", C.c_int64),
+                ("pinned_allocs", C.c_int64), ("pinned_frees", C.c_int64),
+                ("bridge_allocs", C.c_int64), ("bridge_frees", C.c_int64)]
+
+
 class QSAddDesc(C.Structure):
     _fields_ = [
         ("a_type", C.c_int32), ("c_type", C.c_int32), ("compute_type", C.c_int32), ("conj_a", C.c_int32),
@@ -153,6 +159,7 @@ SIGNATURES = {
     "qs_server_destroy": (C.c_int, [C.c_void_p]),
     "qs_fill_offsets": (C.c_int, [C.POINTER(QSGroup), C.c_int64, C.c_int64, C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64)]),
+    "qs_alloc_stats_get": (C.c_int, [C.POINTER(QSAllocStats)]),
 }
 
 _lib = None

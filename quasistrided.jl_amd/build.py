@@ -38,8 +38,17 @@ def sources() -> list[str]:
     return out
 
 
-def _compile(src: str) -> str:
+def _headers_mtime() -> float:
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    paths = [os.path.join(CSRC, n) for n in os.listdir(CSRC) if n.endswith(".h")]
+    paths += [os.path.join(inc, n) for n in os.listdir(inc) if n.endswith(".h")]
+    return max(os.path.getmtime(p) for p in paths)
+
+
+def _compile(src: str, force: bool = True, hdr_mtime: float = 0.0) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if not force and os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(src), hdr_mtime):
+        return obj
     cmd = [HIPCC, *COMMON, "-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC, *COMMON, "-x", "hip", "-c", src, "-o", obj]
@@ -49,13 +58,15 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = True, jobs: int | None = None) -> str:
-    """Compile every source and link libqs_amd.so; returns its path."""
+def build(verbose: bool = True, jobs: int | None = None, force: bool = True) -> str:
+    """Compile every source and link libqs_amd.so; returns its path.  With
+    force=False, objects newer than their source and every header are kept."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
+    hdr = _headers_mtime()
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda src: _compile(src, force, hdr), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -68,4 +79,4 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
 
 
 if __name__ == "__main__":
-    build()
+    build(force="--changed" not in sys.argv)

@@ -234,13 +234,13 @@ qs_status qs_add_plan_create(const qs_add_desc *desc, qs_add_plan **out) {
     fill_offsets(ng, 0, N, nA, nC);
     std::vector<int64_t> unused((size_t)(K > 0 ? K : 1));
     if (K > 0) fill_offsets(kg, 0, K, kA, unused.data());
-    if (hipMalloc(&p->d_tables, sizeof(int64_t) * (size_t)nt) != hipSuccess) {
+    if (dev_malloc(&p->d_tables, sizeof(int64_t) * (size_t)nt) != hipSuccess) {
       (void)hipGetLastError();
       delete p;
       return fail(QS_ERR_MEMORY, "tensoradd!: device tables");
     }
     if (hipMemcpy(p->d_tables, h.data(), sizeof(int64_t) * (size_t)nt, hipMemcpyHostToDevice) != hipSuccess) {
-      (void)hipFree(p->d_tables);
+      (void)dev_free(p->d_tables);
       delete p;
       return fail(QS_ERR_DEVICE, "tensoradd!: table upload");
     }
@@ -321,7 +321,7 @@ qs_status qs_add_plan_rebind(qs_add_plan *p, const void *a, int64_t a_base, int6
 
 qs_status qs_add_plan_destroy(qs_add_plan *p) {
   if (!p) return QS_OK;
-  if (p->d_tables) (void)hipFree(p->d_tables);
+  if (p->d_tables) (void)dev_free(p->d_tables);
   delete p;
   return QS_OK;
 }

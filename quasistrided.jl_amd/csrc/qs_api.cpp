@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -15,14 +17,51 @@
 
 namespace qs {
 
-static thread_local std::string g_last_error;
+// A fixed buffer: setting the message allocates nothing.
+static thread_local char g_last_error[1024];
 
-void set_error(const std::string &msg) { g_last_error = msg; }
+static void set_error_text(const char *msg) { std::snprintf(g_last_error, sizeof g_last_error, "%s", msg); }
+
+void set_error(const std::string &msg) { set_error_text(msg.c_str()); }
 
 qs_status fail(qs_status code, const std::string &msg) {
   set_error(msg);
   return code;
 }
+
+qs_status fail_fmt(qs_status code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(g_last_error, sizeof g_last_error, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+static std::atomic<int64_t> g_dev_allocs{0}, g_dev_frees{0}, g_dev_bytes{0}, g_pin_allocs{0}, g_pin_frees{0};
+static std::atomic<int64_t> g_bridge_allocs{0}, g_bridge_frees{0};
+
+hipError_t dev_malloc(void **p, size_t bytes) {
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e == hipSuccess) {
+    g_dev_allocs.fetch_add(1, std::memory_order_relaxed);
+    g_dev_bytes.fetch_add((int64_t)bytes, std::memory_order_relaxed);
+  }
+  return e;
+}
+hipError_t dev_free(void *p) {
+  if (p) g_dev_frees.fetch_add(1, std::memory_order_relaxed);
+  return hipFree(p);
+}
+hipError_t pinned_malloc(void **p, size_t bytes, unsigned int flags) {
+  const hipError_t e = hipHostMalloc(p, bytes, flags);
+  if (e == hipSuccess) g_pin_allocs.fetch_add(1, std::memory_order_relaxed);
+  return e;
+}
+hipError_t pinned_free(void *p) {
+  if (p) g_pin_frees.fetch_add(1, std::memory_order_relaxed);
+  return hipHostFree(p);
+}
+void count_bridge(bool alloc) { (alloc ? g_bridge_allocs : g_bridge_frees).fetch_add(1, std::memory_order_relaxed); }
 
 static qs_status hip_fail(hipError_t e, const char *what) {
   return fail(QS_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
@@ -150,7 +189,7 @@ static int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Workgroup tile for an M x N x K contraction of compute type ct: enough
 // tiles to cover the 256 CUs, the largest register tile that does so.
-static int choose_tile(int64_t M, int64_t N, int ct) {
+int choose_tile(int64_t M, int64_t N, int ct) {
   const bool cplx = is_complex(ct);
   if (M <= 32) return TILE_32x64;
   const int64_t t128 = cdiv(M, 128) * cdiv(N, 128);
@@ -182,9 +221,40 @@ static double scalar_at(const void *p, int ct, int part) {
   return ((const double *)p)[part];
 }
 
-qs_status plan_build(const qs_contract_desc *desc, const qs_allocator *al, qs_plan **out) {
-  if (!desc || !out) return fail(QS_ERR_ARGUMENT, "qs_plan_create: null argument");
-  const qs_contract_desc &d = *desc;
+// Offset of entry 1 minus entry 0 of map p's table: the stride of the
+// group's first dim longer than one (0 when there is none).
+static int64_t first_step(const qs_group &g, int p) {
+  for (int d = 0; d < g.ndim; ++d)
+    if (g.lengths[d] > 1) return g.strides[p][d];
+  return 0;
+}
+
+// Whether entries 2i and 2i + 1 of map p's table are adjacent in memory for
+// every i (n even): walks the enumeration without storing it.
+static bool pairs_adjacent(const qs_group &g, int p, int64_t n) {
+  if (n % 2 != 0) return false;
+  if (n == 0) return true;
+  int64_t x[QS_MAX_DIMS] = {0};
+  int64_t o = 0, prev = 0;
+  for (int64_t t = 0;;) {
+    if (t % 2 == 1 && o != prev + 1) return false;
+    prev = o;
+    if (++t == n) break;
+    for (int d = 0; d < g.ndim; ++d) {
+      const int64_t L = g.lengths[d];
+      if (x[d] < L - 1) {
+        ++x[d];
+        o += g.strides[p][d];
+        break;
+      }
+      x[d] = 0;
+      o -= (L - 1) * g.strides[p][d];
+    }
+  }
+  return true;
+}
+
+qs_status plan_setup(const qs_contract_desc &d, qs_plan *p) {
   for (int t : {d.a_type, d.b_type, d.c_type, d.compute_type})
     if (t < QS_F32 || t > QS_C64)
       return fail(QS_ERR_ARGUMENT, "plan_contract: eltypes must be Float32, Float64, ComplexF32 or ComplexF64");
@@ -203,11 +273,6 @@ qs_status plan_build(const qs_contract_desc *desc, const qs_allocator *al, qs_pl
   if (d.a_len < 0 || d.b_len < 0 || d.c_len < 0)
     return fail(QS_ERR_ARGUMENT, "plan_contract: negative storage length");
 
-  qs_plan *p = new qs_plan();
-  if (al) {
-    p->al = *al;
-    p->has_al = 1;
-  }
   p->desc = d;
   p->M = group_length(d.m);
   p->N = group_length(d.n);
@@ -238,10 +303,8 @@ qs_status plan_build(const qs_contract_desc *desc, const qs_allocator *al, qs_pl
   else if (M <= 32) p->path = QS_PATH_SKINNY;
   else p->path = QS_PATH_GEMM;
   const bool split = d.precision == QS_SPLIT_INT8 && (p->path == QS_PATH_GEMM || p->path == QS_PATH_SKINNY);
-  if (split && is_complex(ct)) {
-    delete p;
+  if (split && is_complex(ct))
     return fail(QS_ERR_UNSUPPORTED, "plan_contract: split precision is implemented for real compute types only");
-  }
 
   // Coalescing: put a unit-stride dim first in each operand's gather
   // direction.  Reordering M or N dims changes no arithmetic; reordering K
@@ -271,39 +334,60 @@ qs_status plan_build(const qs_contract_desc *desc, const qs_allocator *al, qs_pl
     p->a_kfast = akf;
     p->b_kfast = bkf;
   }
+  p->c_mfast = (M > 1 && first_step(mg, 1) == 1) || !(N > 1 && first_step(ng, 1) == 1);
+  // Paired loads read entries 2i and 2i + 1 of the table each gather is
+  // laid along; both must be adjacent in memory for every i.
+  const bool va = p->a_kfast ? pairs_adjacent(kg, 0, K) : pairs_adjacent(mg, 0, M);
+  const bool vb = p->b_kfast ? pairs_adjacent(kg, 1, K) : pairs_adjacent(ng, 0, N);
+  p->vec = (va && vb) ? 1 : 0;
+  p->tile = TILE_64x64;
+  p->k_splits = 1;
+  p->k_chunk = K;
+  if (split) p->path = QS_PATH_SPLIT;
+  return QS_OK;
+}
+
+// The value of the K tables' pad entries: an offset that puts any row of the
+// operand 2^31 bytes past the start of its span, which the buffer-load
+// gather reads as zero.
+int64_t k_pad_offset(const qs_plan *p, int side) {
+  return side == 0 ? p->kA_lo + (int64_t(1) << 31) / elem_bytes(p->desc.a_type)
+                   : p->kB_lo + (int64_t(1) << 31) / elem_bytes(p->desc.b_type);
+}
+
+qs_status plan_build(const qs_contract_desc *desc, const qs_allocator *al, qs_plan **out) {
+  if (!desc || !out) return fail(QS_ERR_ARGUMENT, "qs_plan_create: null argument");
+  qs_plan *p = new qs_plan();
+  if (al) {
+    p->al = *al;
+    p->has_al = 1;
+  }
+  qs_status st = plan_setup(*desc, p);
+  if (st != QS_OK) {
+    delete p;
+    return st;
+  }
+  const qs_contract_desc &d = p->desc;
+  const int64_t M = p->M, N = p->N, K = p->K;
+  const int ct = d.compute_type;
 
   // Offset tables, built once and kept on the device.  The K tables carry
-  // kKTablePad entries past K whose offsets put any A (B) row 2^31 bytes past
-  // the start of its span: the buffer-load gather reads them as zero, with
-  // no bounds test (its spans stay below 2^31 bytes).
+  // kKTablePad pad entries past K (k_pad_offset): the buffer-load gather
+  // reads them as zero, with no bounds test (its spans stay below 2^31 bytes).
   const int64_t KP = K + kKTablePad;
   const int64_t nt = 2 * (M + N + KP);
   std::vector<int64_t> h((size_t)std::max<int64_t>(nt, 1));
   int64_t *mA = h.data(), *mC = mA + M, *nB = mC + M, *nC = nB + N, *kA = nC + N, *kB = kA + KP;
-  fill_offsets(mg, 0, M, mA, mC);
-  fill_offsets(ng, 0, N, nB, nC);
-  fill_offsets(kg, 0, K, kA, kB);
+  fill_offsets(p->mg, 0, M, mA, mC);
+  fill_offsets(p->ng, 0, N, nB, nC);
+  fill_offsets(p->kg, 0, K, kA, kB);
   for (int64_t k = K; k < KP; ++k) {
-    kA[k] = p->kA_lo + (int64_t(1) << 31) / elem_bytes(d.a_type);
-    kB[k] = p->kB_lo + (int64_t(1) << 31) / elem_bytes(d.b_type);
-  }
-  p->c_mfast = (M > 1 && mC[1] - mC[0] == 1) || !(N > 1 && nC[1] - nC[0] == 1);
-  {
-    // Paired loads read entries 2i and 2i + 1 of the table each gather is
-    // laid along; both must be adjacent in memory for every i.
-    auto pairs = [](const int64_t *t, int64_t n) {
-      if (n % 2 != 0) return false;
-      for (int64_t i = 0; i < n; i += 2)
-        if (t[i + 1] != t[i] + 1) return false;
-      return true;
-    };
-    const bool va = p->a_kfast ? pairs(kA, K) : pairs(mA, M);
-    const bool vb = p->b_kfast ? pairs(kB, K) : pairs(nB, N);
-    p->vec = (va && vb) ? 1 : 0;
+    kA[k] = k_pad_offset(p, 0);
+    kB[k] = k_pad_offset(p, 1);
   }
   hipError_t e = hipSuccess;
   if (nt > 0) {
-    e = hipMalloc(&p->d_tables, sizeof(int64_t) * (size_t)nt);
+    e = dev_malloc(&p->d_tables, sizeof(int64_t) * (size_t)nt);
     if (e == hipSuccess) e = hipMemcpy(p->d_tables, h.data(), sizeof(int64_t) * (size_t)nt, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       qs_status r = e == hipErrorOutOfMemory ? fail(QS_ERR_MEMORY, "plan_contract: device tables") : hip_fail(e, "plan_contract: tables");
@@ -311,20 +395,11 @@ qs_status plan_build(const qs_contract_desc *desc, const qs_allocator *al, qs_pl
       return r;
     }
   }
-  p->t_mA = p->d_tables;
-  p->t_mC = p->t_mA + M;
-  p->t_nB = p->t_mC + M;
-  p->t_nC = p->t_nB + N;
-  p->t_kA = p->t_nC + N;
-  p->t_kB = p->t_kA + KP;
+  set_table_pointers(p, p->d_tables);
 
   // Tile, splits, workspace: an explicit override, else the autotune cache's
   // choice for this shape class, else the heuristic.
-  p->tile = TILE_64x64;
-  p->k_splits = 1;
-  p->k_chunk = K;
-  if (split) {
-    p->path = QS_PATH_SPLIT;
+  if (p->path == QS_PATH_SPLIT) {
     p->moduli = split_moduli_for(d.target_rel_err, ct, K);
     qs_status ss = split_prepare(p);
     if (ss != QS_OK) { qs_plan_destroy(p); return ss; }
@@ -343,8 +418,20 @@ qs_status plan_build(const qs_contract_desc *desc, const qs_allocator *al, qs_pl
   return QS_OK;
 }
 
+// The six tables' positions inside one block: mA mC nB nC kA kB.
+void set_table_pointers(qs_plan *p, const int64_t *t) {
+  const int64_t KP = p->K + kKTablePad;
+  p->t_mA = t;
+  p->t_mC = p->t_mA + p->M;
+  p->t_nB = p->t_mC + p->M;
+  p->t_nC = p->t_nB + p->N;
+  p->t_kA = p->t_nC + p->N;
+  p->t_kB = p->t_kA + KP;
+}
+
 void *ws_alloc(qs_plan *p, size_t bytes) {
   if (p->has_al) {
+    count_bridge(true);
     void *q = p->al.alloc(p->al.ctx, bytes);
     if (q && (uintptr_t)q % 256 != 0) {   // misaligned: handed back unused
       p->al.free(p->al.ctx, q, bytes);
@@ -353,7 +440,7 @@ void *ws_alloc(qs_plan *p, size_t bytes) {
     return q;
   }
   void *q = nullptr;
-  if (hipMalloc(&q, bytes) != hipSuccess) {
+  if (dev_malloc(&q, bytes) != hipSuccess) {
     (void)hipGetLastError();
     return nullptr;
   }
@@ -361,8 +448,12 @@ void *ws_alloc(qs_plan *p, size_t bytes) {
 }
 void ws_free(qs_plan *p, void *ptr, size_t bytes) {
   if (!ptr) return;
-  if (p->has_al) p->al.free(p->al.ctx, ptr, bytes);
-  else (void)hipFree(ptr);
+  if (p->has_al) {
+    count_bridge(false);
+    p->al.free(p->al.ctx, ptr, bytes);
+    return;
+  }
+  (void)dev_free(ptr);
 }
 
 qs_status configure_gemm(qs_plan *p, int tile, int splits) {
@@ -408,7 +499,7 @@ static bool span_ok(int64_t base, int64_t rlo, int64_t rhi, int64_t clo, int64_t
   return lo >= 0 && hi <= (__int128)len - 1;
 }
 
-static qs_status check_spans(qs_plan *p, bool reads_ab) {
+qs_status check_spans(qs_plan *p, bool reads_ab) {
   const qs_contract_desc &d = p->desc;
   if (p->M == 0 || p->N == 0) return QS_OK;
   if (!span_ok(d.c_base, p->mC_lo, p->mC_hi, p->nC_lo, p->nC_hi, d.c_len, p->c_lo, p->c_hi))
@@ -529,7 +620,7 @@ using namespace qs;
 
 extern "C" {
 
-const char *qs_last_error(void) { return g_last_error.c_str(); }
+const char *qs_last_error(void) { return g_last_error; }
 
 const char *qs_version(void) { return "qs_amd 0.1.0 (gfx950)"; }
 
@@ -646,12 +737,12 @@ qs_status qs_plan_destroy(qs_plan *p) {
     ws_free(p, p->d_split, (size_t)p->split_bytes);
     ws_free(p, p->d_partials, (size_t)p->partials_bytes);
   }
-  if (p->d_tables) (void)hipFree(p->d_tables);
+  if (p->d_tables) (void)dev_free(p->d_tables);
   if (p->split_side) (void)hipStreamDestroy(p->split_side);
   if (p->split_fork) (void)hipEventDestroy(p->split_fork);
   if (p->split_join) (void)hipEventDestroy(p->split_join);
   for (int i = 0; i < 3; ++i)
-    if (p->stage[i]) (void)hipFree(p->stage[i]);
+    if (p->stage[i]) (void)dev_free(p->stage[i]);
   delete p;
   return QS_OK;
 }
@@ -670,6 +761,18 @@ qs_status qs_plan_get_error_report(const qs_plan *p, qs_error_report *out) {
   out->rel_fro = p->split_report.rel_fro;
   out->max_rel = p->split_report.max_rel;
   out->target = p->desc.target_rel_err;
+  return QS_OK;
+}
+
+qs_status qs_alloc_stats_get(qs_alloc_stats *out) {
+  if (!out) return fail(QS_ERR_ARGUMENT, "qs_alloc_stats_get: null argument");
+  out->device_allocs = g_dev_allocs.load();
+  out->device_frees = g_dev_frees.load();
+  out->device_bytes = g_dev_bytes.load();
+  out->pinned_allocs = g_pin_allocs.load();
+  out->pinned_frees = g_pin_frees.load();
+  out->bridge_allocs = g_bridge_allocs.load();
+  out->bridge_frees = g_bridge_frees.load();
   return QS_OK;
 }
 

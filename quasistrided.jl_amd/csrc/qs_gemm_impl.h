@@ -479,6 +479,8 @@ struct TileGemm {
       // Wave-uniform offsets (scalar loads) are fetched one stage ahead.
       auto deep = [&](auto ns) {
         constexpr int NS = decltype(ns)::value;
+        // The buffer gather reads the K tables' pad entries past K unguarded.
+        static_assert(k_table_reach(BK, NS) <= kKTablePad, "K table pad shorter than the gather's reach");
         Gather<R, CA, MIXED, BM, BK, AKF, NS, VEC ? 2 : 1> ga;
         Gather<R, CB, MIXED, BN, BK, BKF, NS, VEC ? 2 : 1, BAUX> gb;
         ga.init(t, m0, g.M, g.a_base, g.mA, g.A, g.a_lo, g.a_bytes);
@@ -487,13 +489,13 @@ struct TileGemm {
         typedef decltype(gb) GB;
         auto step = [&](int64_t k0, auto u) {
           constexpr int U = decltype(u)::value, NX = (U + 1) % NS, KL = (U + NS - 1) % NS;
-          if constexpr (!GA::UNIFORM_K) ga.template fetch_k<KL>(k0 + (2 * NS - 1) * BK, kend, g.kA);
-          if constexpr (!GB::UNIFORM_K) gb.template fetch_k<KL>(k0 + (2 * NS - 1) * BK, kend, g.kB);
+          if constexpr (!GA::UNIFORM_K) ga.template fetch_k<KL>(k0 + deep_lane_fetch_ahead(NS) * BK, kend, g.kA);
+          if constexpr (!GB::UNIFORM_K) gb.template fetch_k<KL>(k0 + deep_lane_fetch_ahead(NS) * BK, kend, g.kB);
           __builtin_amdgcn_sched_barrier(0);   // offsets issued before the loads
           ga.template load<U>(g.A, g.a_type);
           gb.template load<U>(g.B, g.b_type);
-          if constexpr (GA::UNIFORM_K) ga.template fetch_k<0>(k0 + (NS + 1) * BK, kend, g.kA);
-          if constexpr (GB::UNIFORM_K) gb.template fetch_k<0>(k0 + (NS + 1) * BK, kend, g.kB);
+          if constexpr (GA::UNIFORM_K) ga.template fetch_k<0>(k0 + deep_uniform_fetch_ahead(NS) * BK, kend, g.kA);
+          if constexpr (GB::UNIFORM_K) gb.template fetch_k<0>(k0 + deep_uniform_fetch_ahead(NS) * BK, kend, g.kB);
           const R *As = smem + (U & 1) * (SA + SB);
           const R *Bs = As + SA;
           R *Ws = smem + ((U + 1) & 1) * (SA + SB);
@@ -588,6 +590,7 @@ struct TileGemm {
       deep(std::integral_constant<int, 2>());
     } else {
       // One stage in flight: stage j + 1 is loaded while stage j multiplies.
+      static_assert(k_table_reach(BK, 1) <= kKTablePad, "K table pad shorter than the gather's reach");
       Gather<R, CA, MIXED, BM, BK, AKF> ga;
       Gather<R, CB, MIXED, BN, BK, BKF> gb;
       ga.init(t, m0, g.M, g.a_base, g.mA, g.A, g.a_lo, g.a_bytes);
@@ -612,8 +615,8 @@ struct TileGemm {
         // past the range is staged but never multiplied.
         ga.template load<0>(g.A, g.a_type);
         gb.template load<0>(g.B, g.b_type);
-        ga.template fetch_k<0>(k0 + 2 * BK, kend, g.kA);
-        gb.template fetch_k<0>(k0 + 2 * BK, kend, g.kB);
+        ga.template fetch_k<0>(k0 + shallow_fetch_ahead() * BK, kend, g.kA);
+        gb.template fetch_k<0>(k0 + shallow_fetch_ahead() * BK, kend, g.kB);
         const R *As = smem + buf * (SA + SB);
         const R *Bs = As + SA;
   #pragma unroll

@@ -146,6 +146,19 @@ struct qs_group_plan {
   double *h_coef = nullptr;             // pinned host staging
   std::vector<double> last_coef;
   int64_t flops = 0;
+  // Each member's storages as the device tables hold them (snapshotted when
+  // the tables were built): a member rebound since then has its tables
+  // rebuilt before the next execute or capture.
+  struct Storages {
+    const void *a, *b;
+    void *c;
+    int64_t a_base, a_len, b_base, b_len, c_base, c_len;
+    bool operator==(const Storages &o) const {
+      return a == o.a && b == o.b && c == o.c && a_base == o.a_base && a_len == o.a_len && b_base == o.b_base &&
+             b_len == o.b_len && c_base == o.c_base && c_len == o.c_len;
+    }
+  };
+  std::vector<Storages> held;
 };
 
 struct qs_graph {
@@ -168,7 +181,7 @@ static int cu_count() {
 
 static void free_launches(qs_group_plan *g) {
   for (auto &l : g->launches)
-    if (l.d_items) (void)hipFree(l.d_items);
+    if (l.d_items) (void)dev_free(l.d_items);
   g->launches.clear();
   g->standalone.clear();
 }
@@ -184,9 +197,16 @@ static int64_t base_c(const qs_group_plan *g, size_t i) {
   return g->plans[i]->desc.c_base + (g->c_off.empty() ? 0 : g->c_off[i]);
 }
 
+static qs_group_plan::Storages storages_of(const qs_plan *p) {
+  const qs_contract_desc &d = p->desc;
+  return qs_group_plan::Storages{d.a, d.b, d.c, d.a_base, d.a_len, d.b_base, d.b_len, d.c_base, d.c_len};
+}
+
 // (Re)build the device tables from the plans and their tile classes.
 static qs_status build_tables(qs_group_plan *g) {
   free_launches(g);
+  g->held.clear();
+  for (const qs_plan *p : g->plans) g->held.push_back(storages_of(p));
   std::map<std::tuple<int, int, int, int>, std::vector<GroupItem>> classes;
   for (size_t i = 0; i < g->plans.size(); ++i) {
     qs_plan *p = g->plans[i];
@@ -229,12 +249,12 @@ static qs_status build_tables(qs_group_plan *g) {
     l.total = start;
     l.grid = (int)std::min<int64_t>(start, 2 * (int64_t)cu_count());
     if (l.grid < 1) l.grid = 1;
-    if (hipMalloc(&l.d_items, sizeof(GroupItem) * v.size()) != hipSuccess) {
+    if (dev_malloc(&l.d_items, sizeof(GroupItem) * v.size()) != hipSuccess) {
       free_launches(g);
       return fail(QS_ERR_MEMORY, "qs_group_create: device table");
     }
     if (hipMemcpy(l.d_items, v.data(), sizeof(GroupItem) * v.size(), hipMemcpyHostToDevice) != hipSuccess) {
-      (void)hipFree(l.d_items);
+      (void)dev_free(l.d_items);
       free_launches(g);
       return fail(QS_ERR_DEVICE, "qs_group_create: table upload");
     }
@@ -296,6 +316,54 @@ static qs_status launch_all(qs_group_plan *g, hipStream_t s) {
   return QS_OK;
 }
 
+// Exact span checks of every member (its plan's storages plus the member's
+// base offsets) before anything is launched or captured.
+static qs_status check_member_spans(const qs_group_plan *g, const char *who) {
+  for (size_t i = 0; i < g->plans.size(); ++i) {
+    const qs_plan *p = g->plans[i];
+    const qs_contract_desc &d = p->desc;
+    const __int128 cb = base_c(g, i), ab = base_a(g, i), bb = base_b(g, i);
+    const __int128 clo = cb + p->mC_lo + p->nC_lo, chi = cb + p->mC_hi + p->nC_hi;
+    const __int128 alo = ab + p->mA_lo + p->kA_lo, ahi = ab + p->mA_hi + p->kA_hi;
+    const __int128 blo = bb + p->kB_lo + p->nB_lo, bhi = bb + p->kB_hi + p->nB_hi;
+    if (clo < 0 || chi >= d.c_len || alo < 0 || ahi >= d.a_len || blo < 0 || bhi >= d.b_len)
+      return fail(QS_ERR_BOUNDS, std::string(who) + ": member " + std::to_string(i) +
+                                     "'s operand span exceeds its storage");
+  }
+  return QS_OK;
+}
+
+static void build_batch_args(qs_group_plan *g);
+
+// Rebuild the device tables when a member plan was rebound since they were
+// built (the tables hold storage pointers and bases).  Work already queued
+// may still read the old tables, so the device is synchronized first.
+static qs_status refresh_tables(qs_group_plan *g) {
+  bool same = g->held.size() == g->plans.size();
+  for (size_t i = 0; same && i < g->plans.size(); ++i) same = g->held[i] == storages_of(g->plans[i]);
+  if (same) return QS_OK;
+  if (hipDeviceSynchronize() != hipSuccess) return fail(QS_ERR_DEVICE, "group: sync before rebuilding tables");
+  qs_status st = build_tables(g);
+  if (st != QS_OK) return st;
+  if (g->d_batch_off) build_batch_args(g);
+  return QS_OK;
+}
+
+// The one batched launch of a batched group: the plan's own kernel over all
+// coordinates, with the offsets table moving the bases.
+static void build_batch_args(qs_group_plan *g) {
+  qs_plan *plan = g->plans[0];
+  GemmArgs a = make_args(plan, plan->desc.a, plan->desc.a_base, plan->desc.b, plan->desc.b_base, plan->desc.c,
+                         plan->desc.c_base);
+  a.k_splits = 1;
+  a.k_chunk = plan->K;
+  a.partials = nullptr;
+  if (g->tile_override[0] >= 0) a.tile = g->tile_override[0];
+  a.batch_off = g->d_batch_off;
+  a.batch = (int64_t)g->plans.size();
+  g->batch_args = a;
+}
+
 // One batch of members for the serving loop: descriptors already built,
 // staged through pinned host buffers owned by the caller's slot.
 qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vector<int> &cls,
@@ -303,16 +371,16 @@ qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vect
   const size_t n = args.size();
   if (n == 0) return QS_OK;
   if (n > (size_t)B.cap) {
-    if (B.d_items) (void)hipFree(B.d_items);
-    if (B.h_items) (void)hipHostFree(B.h_items);
-    if (B.d_coef) (void)hipFree(B.d_coef);
-    if (B.h_coef) (void)hipHostFree(B.h_coef);
+    if (B.d_items) (void)dev_free(B.d_items);
+    if (B.h_items) (void)pinned_free(B.h_items);
+    if (B.d_coef) (void)dev_free(B.d_coef);
+    if (B.h_coef) (void)pinned_free(B.h_coef);
     B.d_items = nullptr; B.h_items = nullptr; B.d_coef = nullptr; B.h_coef = nullptr; B.cap = 0;
     const size_t cap = std::max<size_t>(n, 64);
-    if (hipMalloc((void **)&B.d_items, sizeof(GroupItem) * cap) != hipSuccess ||
-        hipHostMalloc((void **)&B.h_items, sizeof(GroupItem) * cap, hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void **)&B.d_coef, sizeof(double) * 4 * cap) != hipSuccess ||
-        hipHostMalloc((void **)&B.h_coef, sizeof(double) * 4 * cap, hipHostMallocDefault) != hipSuccess)
+    if (dev_malloc((void **)&B.d_items, sizeof(GroupItem) * cap) != hipSuccess ||
+        pinned_malloc((void **)&B.h_items, sizeof(GroupItem) * cap, hipHostMallocDefault) != hipSuccess ||
+        dev_malloc((void **)&B.d_coef, sizeof(double) * 4 * cap) != hipSuccess ||
+        pinned_malloc((void **)&B.h_coef, sizeof(double) * 4 * cap, hipHostMallocDefault) != hipSuccess)
       return fail(QS_ERR_MEMORY, "serving: batch buffers");
     B.cap = (int)cap;
   }
@@ -392,7 +460,7 @@ qs_status qs_group_create(qs_plan *const *plans, int32_t n, const int32_t *tiles
   if (tiles) g->tile_override.assign(tiles, tiles + n);
   for (qs_plan *p : g->plans) g->flops += (is_complex(p->desc.compute_type) ? 8 : 2) * p->M * p->N * p->K;
   const size_t cb = sizeof(double) * 4 * (size_t)std::max<int32_t>(n, 1);
-  if (hipMalloc(&g->d_coef, cb) != hipSuccess || hipHostMalloc(&g->h_coef, cb, hipHostMallocDefault) != hipSuccess) {
+  if (dev_malloc(&g->d_coef, cb) != hipSuccess || pinned_malloc(&g->h_coef, cb, hipHostMallocDefault) != hipSuccess) {
     qs_group_destroy(g);
     return fail(QS_ERR_MEMORY, "qs_group_create: coefficient buffers");
   }
@@ -421,9 +489,15 @@ qs_status qs_group_create_batched(qs_plan *plan, int32_t n, const int64_t *a_off
   g->c_off.assign(c_offsets, c_offsets + n);
   g->flops = (int64_t)n * (is_complex(plan->desc.compute_type) ? 8 : 2) * plan->M * plan->N * plan->K;
   const size_t cb = sizeof(double) * 4 * (size_t)std::max<int32_t>(n, 1);
-  if (hipMalloc(&g->d_coef, cb) != hipSuccess || hipHostMalloc(&g->h_coef, cb, hipHostMallocDefault) != hipSuccess) {
+  if (dev_malloc(&g->d_coef, cb) != hipSuccess || pinned_malloc(&g->h_coef, cb, hipHostMallocDefault) != hipSuccess) {
     qs_group_destroy(g);
     return fail(QS_ERR_MEMORY, "qs_group_create_batched: coefficient buffers");
+  }
+  // Every coordinate's spans are checked before anything is built.
+  st = check_member_spans(g, "qs_group_create_batched");
+  if (st != QS_OK) {
+    qs_group_destroy(g);
+    return st;
   }
   st = build_tables(g);
   if (st != QS_OK) {
@@ -437,20 +511,12 @@ qs_status qs_group_create_batched(qs_plan *plan, int32_t n, const int64_t *a_off
       off[3 * i + 1] = b_offsets[i];
       off[3 * i + 2] = c_offsets[i];
     }
-    if (hipMalloc(&g->d_batch_off, sizeof(int64_t) * off.size()) != hipSuccess ||
+    if (dev_malloc(&g->d_batch_off, sizeof(int64_t) * off.size()) != hipSuccess ||
         hipMemcpy(g->d_batch_off, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice) != hipSuccess) {
       qs_group_destroy(g);
       return fail(QS_ERR_MEMORY, "qs_group_create_batched: offset table");
     }
-    GemmArgs a = make_args(plan, plan->desc.a, plan->desc.a_base, plan->desc.b, plan->desc.b_base, plan->desc.c,
-                           plan->desc.c_base);
-    a.k_splits = 1;
-    a.k_chunk = plan->K;
-    a.partials = nullptr;
-    if (tile >= 0) a.tile = tile;
-    a.batch_off = g->d_batch_off;
-    a.batch = n;
-    g->batch_args = a;
+    build_batch_args(g);
   }
   *out = g;
   return QS_OK;
@@ -460,17 +526,10 @@ qs_status qs_group_execute(qs_group_plan *g, const double *alphas, const double 
   if (!g) return fail(QS_ERR_ARGUMENT, "qs_group_execute: null group");
   hipStream_t s = (hipStream_t)stream;
   // Exact span checks for every member before anything is launched.
-  for (size_t i = 0; i < g->plans.size(); ++i) {
-    const qs_plan *p = g->plans[i];
-    const qs_contract_desc &d = p->desc;
-    const int64_t cb = base_c(g, i), ab = base_a(g, i), bb = base_b(g, i);
-    const int64_t clo = cb + p->mC_lo + p->nC_lo, chi = cb + p->mC_hi + p->nC_hi;
-    const int64_t alo = ab + p->mA_lo + p->kA_lo, ahi = ab + p->mA_hi + p->kA_hi;
-    const int64_t blo = bb + p->kB_lo + p->nB_lo, bhi = bb + p->kB_hi + p->nB_hi;
-    if (clo < 0 || chi >= d.c_len || alo < 0 || ahi >= d.a_len || blo < 0 || bhi >= d.b_len)
-      return fail(QS_ERR_BOUNDS, "qs_group_execute: a member's operand span exceeds its storage");
-  }
-  qs_status st = set_coefficients(g, alphas, betas, s);
+  qs_status st = check_member_spans(g, "qs_group_execute");
+  if (st != QS_OK) return st;
+  if ((st = refresh_tables(g)) != QS_OK) return st;
+  st = set_coefficients(g, alphas, betas, s);
   if (st != QS_OK) return st;
   return launch_all(g, s);
 }
@@ -484,9 +543,9 @@ qs_status qs_group_flops(const qs_group_plan *g, int64_t *flops) {
 qs_status qs_group_destroy(qs_group_plan *g) {
   if (!g) return QS_OK;
   free_launches(g);
-  if (g->d_coef) (void)hipFree(g->d_coef);
-  if (g->h_coef) (void)hipHostFree(g->h_coef);
-  if (g->d_batch_off) (void)hipFree(g->d_batch_off);
+  if (g->d_coef) (void)dev_free(g->d_coef);
+  if (g->h_coef) (void)pinned_free(g->h_coef);
+  if (g->d_batch_off) (void)dev_free(g->d_batch_off);
   delete g;
   return QS_OK;
 }
@@ -497,8 +556,14 @@ qs_status qs_group_capture(qs_group_plan *g, const double *alphas, const double 
   *out = nullptr;
   if (repeats < 1) return fail(QS_ERR_ARGUMENT, "qs_group_capture: repeats must be >= 1");
   hipStream_t s = (hipStream_t)stream;
+  // The same exact span checks as an execute, before anything is captured:
+  // the graph replays these storages (a member rebound later needs a new
+  // capture).
+  qs_status st = check_member_spans(g, "qs_group_capture");
+  if (st != QS_OK) return st;
+  if ((st = refresh_tables(g)) != QS_OK) return st;
   // Coefficients are uploaded before capture: the graph holds kernel nodes only.
-  qs_status st = set_coefficients(g, alphas, betas, s);
+  st = set_coefficients(g, alphas, betas, s);
   if (st != QS_OK) return st;
   hipStream_t cs;
   if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)

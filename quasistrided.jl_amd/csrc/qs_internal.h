@@ -8,12 +8,27 @@
 #include <string>
 
 #include "../../include/qs_amd.h"
+#include "qs_tiles.h"
 
 namespace qs {
 
 // Thread-local error text behind qs_last_error().
 void set_error(const std::string &msg);
 qs_status fail(qs_status code, const std::string &msg);
+
+// Every device and pinned-host allocation the library makes goes through
+// these, which count them (qs_alloc_stats_get): a steady-state execute must
+// leave the counts unchanged.
+hipError_t dev_malloc(void **p, size_t bytes);
+template <typename T> hipError_t dev_malloc(T **p, size_t bytes) { return dev_malloc((void **)p, bytes); }
+hipError_t dev_free(void *p);
+hipError_t pinned_malloc(void **p, size_t bytes, unsigned int flags);
+template <typename T> hipError_t pinned_malloc(T **p, size_t bytes, unsigned int flags) {
+  return pinned_malloc((void **)p, bytes, flags);
+}
+hipError_t pinned_free(void *p);
+// Workspace blocks taken from / handed back to a caller's qs_allocator.
+void count_bridge(bool alloc);
 
 inline bool is_complex(int t) { return t == QS_C32 || t == QS_C64; }
 inline bool is_double(int t) { return t == QS_F64 || t == QS_C64; }
@@ -70,31 +85,6 @@ struct GemmArgs {
   // loads (a_lo / a_bytes and b_lo / b_bytes are set).
   int dom_buf = 0;
 };
-
-// Row pitch, in elements, of an LDS tile stored [row][k]: the smallest pitch
-// >= bk that keeps a fragment read free of bank conflicts.
-constexpr int lds_pitch_rowmajor(int rsize, int bk) { return rsize == 8 ? bk + 1 : ((bk / 4) % 2 ? bk : bk + 4); }
-// K depth of one LDS stage: the largest power of two in [4, 32] (4-byte
-// elements: [4, 64]) whose double-buffered A and B tiles fit the LDS budget
-// (76 KiB: two workgroups per CU).  FP32 MFMAs run at twice the FP64 rate, so
-// a stage's fixed costs weigh twice as much; a deeper stage halves them.
-// Shared by the kernels and the host planner, which rounds split-K chunks to it.
-constexpr int pick_bk_for(int rsize, int planes, int bm, int bn, bool akf, bool bkf) {
-  int best = 4;
-  for (int bk = 4; bk <= (rsize == 4 ? 64 : 32); bk *= 2) {
-    const int a = akf ? bm * lds_pitch_rowmajor(rsize, bk) : bk * (bm + 16);
-    const int b = bkf ? bn * lds_pitch_rowmajor(rsize, bk) : bk * (bn + 16);
-    if (2 * planes * (a + b) * rsize <= 76 * 1024) best = bk;
-  }
-  return best;
-}
-
-// Largest operand span, in bytes, that the buffer-load gather addresses: a
-// K-table pad entry moves an offset 2^31 bytes on, past any such span.
-constexpr int64_t kBufferSpanMax = 0x7FFFFF00LL;
-// Entries past K in the K offset tables: the gather fetches offsets up to
-// 191 K steps past the last one it multiplies (three stages of 64, five of 32).
-constexpr int64_t kKTablePad = 256;
 
 // Launch the tiled MFMA GEMM (with k_splits > 1, a second kernel reduces the
 // partials in a fixed order).  Shapes and tables must already be validated.

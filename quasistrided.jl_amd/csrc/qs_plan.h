@@ -49,6 +49,19 @@ struct qs_plan {
 };
 
 namespace qs {
+// Everything a plan decides without allocating: validation, groups in device
+// order, path, gather directions, span ranges, paired-load flag.
+qs_status plan_setup(const qs_contract_desc &d, qs_plan *p);
+// Point the plan's six table pointers into one block laid out mA mC nB nC kA kB.
+void set_table_pointers(qs_plan *p, const int64_t *tables);
+// Value of the K tables' pad entries for A (side 0) or B (side 1).
+int64_t k_pad_offset(const qs_plan *p, int side);
+// The heuristic workgroup tile.
+int choose_tile(int64_t M, int64_t N, int ct);
+// Exact span checks of the plan's current storages (tiles.jl:123-134).
+qs_status check_spans(qs_plan *p, bool reads_ab);
+// An error message formatted into the thread's fixed buffer (no allocation).
+qs_status fail_fmt(qs_status code, const char *fmt, ...);
 // The plan's device workspace, from its allocator (or hipMalloc / hipFree).
 void *ws_alloc(qs_plan *p, size_t bytes);
 void ws_free(qs_plan *p, void *ptr, size_t bytes);

@@ -380,10 +380,10 @@ qs_status qs_server_destroy(qs_server *s) {
     }
     if (s->events[i]) (void)hipEventDestroy(s->events[i]);
     BatchBuffers &B = s->bufs[i];
-    if (B.d_items) (void)hipFree(B.d_items);
-    if (B.h_items) (void)hipHostFree(B.h_items);
-    if (B.d_coef) (void)hipFree(B.d_coef);
-    if (B.h_coef) (void)hipHostFree(B.h_coef);
+    if (B.d_items) (void)dev_free(B.d_items);
+    if (B.h_items) (void)pinned_free(B.h_items);
+    if (B.d_coef) (void)dev_free(B.d_coef);
+    if (B.h_coef) (void)pinned_free(B.h_coef);
   }
   delete s;
   return QS_OK;

@@ -14,10 +14,10 @@ namespace qs {
 
 static qs_status ensure(qs_plan *p, int i, size_t bytes) {
   if (bytes <= p->stage_cap[i]) return QS_OK;
-  if (p->stage[i]) (void)hipFree(p->stage[i]);
+  if (p->stage[i]) (void)dev_free(p->stage[i]);
   p->stage[i] = nullptr;
   p->stage_cap[i] = 0;
-  hipError_t e = hipMalloc(&p->stage[i], bytes);
+  hipError_t e = dev_malloc(&p->stage[i], bytes);
   if (e != hipSuccess) return fail(QS_ERR_MEMORY, std::string("execute!: staging buffer: ") + hipGetErrorString(e));
   p->stage_cap[i] = bytes;
   return QS_OK;

@@ -93,7 +93,7 @@ qs_status qs_autotune(qs_plan *p, void *stream, int32_t reps, qs_tune_result *ou
     return fail(QS_ERR_BOUNDS, "autotune: operand spans exceed their storages");
   void *scratch = nullptr;
   const size_t bytes = (size_t)(chi - clo + 1) * elem_bytes(p->desc.c_type);
-  hipError_t e = hipMalloc(&scratch, bytes);
+  hipError_t e = dev_malloc(&scratch, bytes);
   if (e != hipSuccess) return fail(QS_ERR_MEMORY, "autotune: scratch C");
   const int64_t c_base = p->desc.c_base - clo;
   hipStream_t s = (hipStream_t)stream;
@@ -129,7 +129,7 @@ qs_status qs_autotune(qs_plan *p, void *stream, int32_t reps, qs_tune_result *ou
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipFree(scratch);
+  (void)dev_free(scratch);
   if (st != QS_OK) {
     configure_gemm(p, tile0, splits0);
     return st;

@@ -124,6 +124,7 @@ class ContractPlan:
         self._views = views  # keeps every operand alive while the plan lives
         self._al_keep = None  # the allocator bridge, kept until the plan is destroyed
         self._cached = False  # owned by the layout cache
+        self.desc = None  # the qs_contract_desc the device plan was built from
         info = _lib.QSPlanInfo()
         st = _lib.lib().qs_plan_get_info(handle, C.byref(info))
         if st != 0:
@@ -305,6 +306,7 @@ def planned(C_, A_, indA, B_, indB, indC, *, kernel=None, conjA=False, conjB=Fal
         plan = ContractPlan(handle, kern, mgroup, ngroup, kgroup, Av.parent, Av.offset, Bv.parent, Bv.offset,
                             Cv.parent, Cv.offset, atransform, btransform, T, swapped, views)
         plan._cached = True
+        plan.desc = d
         return plan
     handle = C.c_void_p()
     keep = None
@@ -318,6 +320,7 @@ def planned(C_, A_, indA, B_, indB, indC, *, kernel=None, conjA=False, conjB=Fal
     plan = ContractPlan(handle, kern, mgroup, ngroup, kgroup, Av.parent, Av.offset, Bv.parent, Bv.offset,
                         Cv.parent, Cv.offset, atransform, btransform, T, swapped, (Av, Bv, Cv))
     plan._al_keep = keep
+    plan.desc = d   # the qs_contract_desc the device plan was built from
     return plan
 
 

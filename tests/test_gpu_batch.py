@@ -152,9 +152,10 @@ def test_batch_spans_checked_before_any_write():
     Bst = randn(rng, np.float64, k * j * b)
     Bv = dev_view(Bst, (k, j, b), (1, k, k * j + 1))   # the last batch slice runs past the storage
     Cv = dev_view(np.zeros(i * j * b), (i, j, b), colmajor((i, j, b)))
-    bp = qs.plan_batched(Cv, Av, (1, 2, 3), Bv, (2, 4, 3), (1, 4, 3))
+    # every coordinate's spans are checked when the batched group is built
     with pytest.raises(qs.BoundsError):
-        bp.execute(1.0, 0.0)
+        qs.plan_batched(Cv, Av, (1, 2, 3), Bv, (2, 4, 3), (1, 4, 3))
+    torch.cuda.synchronize()
     assert np.all(values(Cv) == 0)
 
 
