@@ -374,6 +374,20 @@ qs_status qs_server_stats_get(qs_server *s, qs_server_stats *out);
 qs_status qs_server_reset_stats(qs_server *s);
 qs_status qs_server_destroy(qs_server *s);
 
+/* ---- a contraction that allocates nothing --------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, serving loops and harnesses that
+ * forbid hipMalloc.  Plans and runs `desc` in one call (qs_plan_create +
+ * qs_execute + qs_plan_destroy) with no device or host allocation on the
+ * success path: the offset tables are written by a kernel on `stream` into a
+ * fixed array of 4 MiB per GPU, so M + N + K <= 261824 (larger groups:
+ * QS_ERR_UNSUPPORTED), and K runs unsplit.  Device memory only; split
+ * precision is refused (QS_ERR_UNSUPPORTED).  Same validation, span checks
+ * and kernels as qs_execute: results are bitwise those of an unsplit plan.
+ * Every call on one GPU shares that array and leaves its tables there: calls
+ * that overlap on one GPU (two streams, two threads) race on it. */
+qs_status qs_contract_noalloc(const qs_contract_desc *desc, const void *alpha, const void *beta, void *stream);
+
 /* ---- allocation counts ----------------------------------------------------
  * No counterpart in the reference, whose steady-state `execute!` allocates
  * nothing (test/execution/test_execute.jl:72-94).  Every device and pinned

@@ -160,6 +160,7 @@ SIGNATURES = {
     "qs_fill_offsets": (C.c_int, [C.POINTER(QSGroup), C.c_int64, C.c_int64, C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64)]),
     "qs_alloc_stats_get": (C.c_int, [C.POINTER(QSAllocStats)]),
+    "qs_contract_noalloc": (C.c_int, [C.POINTER(QSContractDesc), C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
