@@ -361,6 +361,7 @@ def bench_grouped(qs, torch, dev, cplx, reps=5):
     torch.cuda.synchronize()
     t_plain = (time.perf_counter() - t0) / reps
     grp = qs.GroupPlan(plans)
+    tune = grp.autotune(reps=3)
     grp.execute(1.0, 0.0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -387,6 +388,7 @@ def bench_grouped(qs, torch, dev, cplx, reps=5):
         "graph_vs_per_launch": round(t_plain / t_graph, 2),
         "graph_tflops": round(flops / t_graph / 1e12, 2),
         "graph_mfma_frac": round(flops / t_graph / PEAK_FP64, 4),
+        "autotune": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tune.items()},
     }
     graph.release()
     grp.release()
@@ -490,6 +492,21 @@ def bench_saturated(qs, torch, dev, pools, grouped, n=3000, seed=6, small_share=
     return {"mode": "grouped" if grouped else "per-request launch", "requests": n,
             "throughput_req_per_s": round(st["throughput_rps"], 1), "tflops": round(st["tflops"], 2),
             "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 2)}
+
+
+def bench_plain_extra(qs, torch, dev, key):
+    """The other plain shapes: rank-4 all-64 (M = N = K = 4096), the 48/96
+    C[a,n,b] shape and the ComplexF64 permuted all-32 one, each autotuned."""
+    maker = {"rank4_f64": lambda: config4(qs, torch, dev), "c_anb_f64": lambda: config1(qs, torch, dev),
+             "permuted_c64": lambda: config2(qs, torch, np.complex128, dev)}[key]
+    p, fl, by, vw, be = maker()
+    qs.autotune(p, reps=3)
+    steps = 20 if key.startswith("rank4") else 100
+    w, km, _ = time_plan(qs, torch, p, be, steps, 3)
+    out = {"gflops": round(fl * steps / w / 1e9, 1), "kernel_ms": round(km * 1e3, 4),
+           "mfma_frac": round(fl / km / PEAK_FP64, 4), "plan": repr(p)}
+    del p, vw
+    return out
 
 
 def time_plan(qs, torch, plan, beta, steps, warmup):
@@ -598,7 +615,8 @@ def main():
     ap.add_argument("--tune-cache", default=None,
                     help="autotune cache file: loaded if it exists (no tuning runs), else written after tuning")
     ap.add_argument("--only", default=None,
-                    choices=["split", "grouped_f64", "grouped_c64", "served", "bandwidth", "batched", "mixed"],
+                    choices=["split", "grouped_f64", "grouped_c64", "served", "bandwidth", "batched", "mixed",
+                             "rank4_f64", "c_anb_f64", "permuted_c64", "peaks"],
                     help="only run one of the extra workloads and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy 5 times (PMC byte-counter calibration)")
@@ -635,6 +653,10 @@ def main():
             out = bench_mixed_domain(qs, torch, dev)
         elif args.only.startswith("grouped"):
             out = bench_grouped(qs, torch, dev, args.only.endswith("c64"))
+        elif args.only == "peaks":
+            out = qs.measure_peaks()
+        elif args.only in ("rank4_f64", "c_anb_f64", "permuted_c64"):
+            out = bench_plain_extra(qs, torch, dev, args.only)
         else:
             pools = served_pools(qs, torch, dev)
             out = {"grouped": bench_served(qs, torch, dev, pools, True),
@@ -718,16 +740,9 @@ def main():
         }
     if not args.no_extra:
         extra = {}
-        for name, maker in (("rank4_all64_f64", lambda: config4(qs, torch, dev)),
-                            ("c_anb_48x96_f64", lambda: config1(qs, torch, dev)),
-                            ("permuted_all32_c64", lambda: config2(qs, torch, np.complex128, dev))):
-            p, fl, by, vw, be = maker()
-            qs.autotune(p, reps=3)
-            steps = 20 if name.startswith("rank4") else 100
-            w, km, _ = time_plan(qs, torch, p, be, steps, 3)
-            extra[name] = {"gflops": round(fl * steps / w / 1e9, 1), "kernel_ms": round(km * 1e3, 4),
-                           "mfma_frac": round(fl / km / PEAK_FP64, 4), "plan": repr(p)}
-            del p, vw
+        for name, key in (("rank4_all64_f64", "rank4_f64"), ("c_anb_48x96_f64", "c_anb_f64"),
+                          ("permuted_all32_c64", "permuted_c64")):
+            extra[name] = bench_plain_extra(qs, torch, dev, key)
         extra["fp32"] = bench_fp32(qs, torch, dev)
         extra["split_rank4_all64_f64"] = bench_split(qs, torch, dev)
         extra["bandwidth_f64"] = bench_bandwidth(qs, torch, dev)

@@ -183,6 +183,14 @@ const char *qs_version(void);
  * (src/hardware/target.jl:181-191); never fails on a gfx950 device. */
 qs_status qs_target_profile_get(qs_target_profile *out);
 
+/* Measured peaks of the current device, reported beside the datasheet ones
+ * (the roofline is priced against both): the FP64 MFMA rate that independent
+ * chains of v_mfma_f64_16x16x4_f64 sustain on every SIMD, in TFLOP/s, and the
+ * rate of a device-to-device copy of 1 GiB, in GB/s (read + write bytes).
+ * Allocates 2 GiB of device memory for the copy while it runs.  Counterpart
+ * of the facts `target_profile()` detects (src/hardware/target.jl:181-191). */
+qs_status qs_target_measure(double *fp64_tflops, double *copy_gbs, void *stream);
+
 /* Validate `desc` and build a plan: the axis-group overflow policy
  * (axis_group.jl:27-41), eltype/compute-type rules (plan.jl:278-295), offset
  * tables (fill_offsets!, axis_group.jl:87-145) uploaded once, the exact span of
@@ -262,8 +270,28 @@ typedef struct qs_error_report {
   double rel_fro;     /* sampled relative Frobenius error       */
   double max_rel;     /* largest sampled relative error         */
   double target;      /* the requested error (0 = default)      */
+  int32_t exceeded;   /* 1: the last execute's sampled error (of the result
+                         delivered) is above the target               */
+  int32_t escalations;/* guarded mode: reruns with more moduli in the last execute */
+  int32_t fallback;   /* guarded mode: 1 if the last execute was finished by
+                         the plain FP64 / FP32 MFMA kernel              */
+  int32_t _pad;
 } qs_error_report;
 
+/* The error bound.  The moduli count makes every entry satisfy
+ * |C_ij - C_exact,ij| <= target * ||A_i||_2 ||B_j||_2 (row i of op(A) and
+ * column j of op(B) over K; plus the final rounding), a normwise bound, like
+ * the reference's norm-wise tolerance of mixed-precision results
+ * (test/execution/test_mixed.jl:50).  For entries of random sign that is an
+ * error about target / (2 sqrt(K)) of ||A_i|| ||B_j||, i.e. about target
+ * relative to C.  Under cancellation (|C_ij| << ||A_i|| ||B_j||) the error
+ * relative to C grows by the cancellation factor, as plain FP64's does (by
+ * about 7x less at K = 4096).
+ * `on`: 0 off; 1 every execute evaluates the report; 2 guarded -- an execute
+ * whose sampled error relative to C exceeds the target is redone (from C as
+ * it was, kept in the workspace when beta != 0) with more moduli, up to 15,
+ * and then with the plain MFMA kernel; the raised moduli count stays for
+ * later executes. */
 qs_status qs_plan_set_error_report(qs_plan *plan, int32_t on);
 qs_status qs_plan_get_error_report(const qs_plan *plan, qs_error_report *out);
 
@@ -326,6 +354,30 @@ qs_status qs_group_create_batched(qs_plan *plan, int32_t n, const int64_t *a_off
  * two doubles per member (re, im; im ignored for real members); NULL means
  * alpha = 1 / beta = 0.  Spans of every member are checked before launch. */
 qs_status qs_group_execute(qs_group_plan *g, const double *alphas, const double *betas, void *stream);
+
+/* Per-shape-class tiles for grouped launches.  No counterpart in the
+ * reference (whose register tile is chosen per shape from its profile,
+ * src/planning/kernel_selection.jl:232-256).  The whole group runs (into a
+ * scratch C: the members' C is not touched) once with every member on each
+ * grouped tile class, the kernel timing each member's tiles inside that mixed
+ * launch; per shape class (types, gather directions, octave buckets of M, N
+ * and K) the tile whose members took the least time wins.  The tuned
+ * assignment is kept only if the whole group runs faster with it than with
+ * the padding heuristic.  Chosen tiles go to the autotune cache, which
+ * qs_group_create consults and qs_autotune_save / qs_autotune_load persist,
+ * and the group's tables are rebuilt.  Results do not change (every tile
+ * sums K in the same order).  `reps` timed runs per whole-group timing
+ * (0 = 3). */
+typedef struct qs_group_tune_result {
+  int32_t classes;          /* shape classes tuned                          */
+  int32_t candidates;       /* tile classes tried (whole-group runs)        */
+  int32_t members_changed;  /* members whose tile differs from before       */
+  int32_t _pad;
+  double before_ms;         /* the whole group with the padding heuristic's tiles */
+  double after_ms;          /* the whole group with the tiles kept          */
+} qs_group_tune_result;
+
+qs_status qs_group_autotune(qs_group_plan *g, void *stream, int32_t reps, qs_group_tune_result *out);
 
 /* Algorithmic flops of one execute (2 per real MAC, 8 per complex MAC). */
 qs_status qs_group_flops(const qs_group_plan *g, int64_t *flops);

@@ -15,7 +15,7 @@ from .plan import (ContractPlan, MFMAKernel, PathModes, autotune, autotune_clear
                    release, set_error_report)
 from .backend import QuasiStridedBackend, contraction_labels, tensorcontract
 from .add import tensoradd, tensortrace
-from .target import target_profile
+from .target import measure_peaks, target_profile
 from .group import GroupPlan, Graph
 from .batch import BatchedPlan, batch_labels, contract_batched, plan_batched
 from .server import Server
@@ -30,6 +30,6 @@ __all__ = [
     "PathModes", "autotune", "autotune_clear", "autotune_load", "autotune_save", "compute_type", "contract", "contract_", "error_report", "set_error_report", "execute", "execute_", "isconj", "op_conjugates",
     "plan_contract", "planned", "release", "QuasiStridedBackend", "contraction_labels", "tensorcontract",
     "tensoradd", "tensortrace",
-    "target_profile", "GroupPlan", "Graph", "Server", "BatchedPlan", "batch_labels", "contract_batched",
+    "target_profile", "measure_peaks", "GroupPlan", "Graph", "Server", "BatchedPlan", "batch_labels", "contract_batched",
     "plan_batched", "DefaultAllocator", "TorchAllocator", "ncon", "ncon_steps", "optimal_order", "contraction_cost",
 ]

@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libqs_amd.so")
+# The library next to this file; QS_AMD_LIB names another build of it.
+LIB_PATH = os.environ.get("QS_AMD_LIB") or os.path.join(HERE, "libqs_amd.so")
 
 QS_MAX_DIMS = 16
 
@@ -81,7 +82,8 @@ class QSServerStats(C.Structure):
 
 class QSErrorReport(C.Structure):
     _fields_ = [("moduli", C.c_int32), ("samples", C.c_int32), ("rel_fro", C.c_double), ("max_rel", C.c_double),
-                ("target", C.c_double)]
+                ("target", C.c_double), ("exceeded", C.c_int32), ("escalations", C.c_int32),
+                ("fallback", C.c_int32), ("_pad", C.c_int32)]
 
 
 class QSTargetProfile(C.Structure):
@@ -97,6 +99,11 @@ class QSTargetProfile(C.Structure):
         ("peak_fp64_tflops", C.c_double), ("peak_fp32_tflops", C.c_double),
         ("peak_int8_tops", C.c_double), ("peak_hbm_gbs", C.c_double),
     ]
+
+
+class QSGroupTuneResult(C.Structure):
+    _fields_ = [("classes", C.c_int32), ("candidates", C.c_int32), ("members_changed", C.c_int32),
+                ("_pad", C.c_int32), ("before_ms", C.c_double), ("after_ms", C.c_double)]
 
 
 class QSAllocStats(C.Structure):
@@ -119,6 +126,7 @@ SIGNATURES = {
     "qs_last_error": (C.c_char_p, []),
     "qs_version": (C.c_char_p, []),
     "qs_target_profile_get": (C.c_int, [C.POINTER(QSTargetProfile)]),
+    "qs_target_measure": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]),
     "qs_plan_create": (C.c_int, [C.POINTER(QSContractDesc), C.POINTER(C.c_void_p)]),
     "qs_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "qs_plan_get_info": (C.c_int, [C.c_void_p, C.POINTER(QSPlanInfo)]),
@@ -142,6 +150,7 @@ SIGNATURES = {
                                           C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_void_p)]),
     "qs_group_execute": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]),
     "qs_group_flops": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "qs_group_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(QSGroupTuneResult)]),
     "qs_group_destroy": (C.c_int, [C.c_void_p]),
     "qs_group_capture": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_void_p,
                                    C.POINTER(C.c_void_p)]),

@@ -166,6 +166,42 @@ int unit_dim(const qs_group &g, int p) {
   return -1;
 }
 
+// --------------------------------------------------------- device facts
+
+// XCDs of a part: not a device property; the CDNA3/4 parts deal workgroups
+// over 8 XCDs (gfx950: 8 x 32 CUs; gfx942: 38 CUs each, 6 on MI300A).
+static int xcds_of(const char *arch, int cus) {
+  if (std::strncmp(arch, "gfx950", 6) == 0) return 8;
+  if (std::strncmp(arch, "gfx942", 6) == 0) return cus % 38 == 0 ? cus / 38 : 8;
+  return 1;
+}
+
+const DeviceFacts &device_facts() {
+  static std::mutex mu;
+  static DeviceFacts facts[64];
+  static bool have[64] = {false};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!have[dev]) {
+    DeviceFacts f;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, dev) == hipSuccess) {
+      std::snprintf(f.arch, sizeof f.arch, "%s", pr.gcnArchName);
+      if (pr.multiProcessorCount > 0) f.cus = pr.multiProcessorCount;
+      if (pr.warpSize > 0) f.wave = pr.warpSize;
+      if (pr.maxSharedMemoryPerMultiProcessor > 0) f.lds_per_cu = (int)pr.maxSharedMemoryPerMultiProcessor;
+      if (pr.sharedMemPerBlock > 0) f.lds_per_block = (int)pr.sharedMemPerBlock;
+      f.xcds = xcds_of(f.arch, f.cus);
+    } else {
+      (void)hipGetLastError();
+    }
+    facts[dev] = f;
+    have[dev] = true;
+  }
+  return facts[dev];
+}
+
 // ------------------------------------------------------------------ tiles
 
 // Mirrors pick_bk_for in qs_gemm_impl.h.
@@ -188,22 +224,25 @@ TileShape tile_shape(int tile, int ct, bool akf, bool bkf) {
 static int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Workgroup tile for an M x N x K contraction of compute type ct: enough
-// tiles to cover the 256 CUs, the largest register tile that does so.
+// tiles to cover the device's CUs, the largest register tile that does so.
 int choose_tile(int64_t M, int64_t N, int ct) {
   const bool cplx = is_complex(ct);
+  const int64_t cus = device_facts().cus;
   if (M <= 32) return TILE_32x64;
   const int64_t t128 = cdiv(M, 128) * cdiv(N, 128);
-  if (!cplx && t128 >= 256) return TILE_128x128;
+  if (!cplx && t128 >= cus) return TILE_128x128;
   const int64_t t12864 = cdiv(M, 128) * cdiv(N, 64);
-  if (t12864 >= 256) return TILE_128x64;
+  if (t12864 >= cus) return TILE_128x64;
   return TILE_64x64;
 }
 
-// Fixed-order split-K when the tile grid alone leaves CUs idle.
+// Fixed-order split-K when the tile grid alone leaves CUs idle: up to two
+// workgroups per CU in all.
 static int choose_splits(int64_t tiles, int64_t K, int bk, int64_t &k_chunk) {
+  const int64_t cus = device_facts().cus;
   int s = 1;
-  if (tiles < 256 && K >= 512) {
-    s = (int)std::min<int64_t>(cdiv(512, tiles), K / 256);
+  if (tiles < cus && K >= 512) {
+    s = (int)std::min<int64_t>(cdiv(2 * cus, tiles), K / 256);
     s = std::max(s, 1);
     s = std::min(s, 16);
   }
@@ -249,6 +288,30 @@ static bool pairs_adjacent(const qs_group &g, int p, int64_t n) {
       }
       x[d] = 0;
       o -= (L - 1) * g.strides[p][d];
+    }
+  }
+  return true;
+}
+
+// The step of map p if it is a single ramp, offset(q) == q * step for every
+// coordinate q (map_ramp_step, src/layout/axis_group.jl:149-168; an empty
+// group is a ramp of step 0).
+static bool map_ramp_step(const qs_group &g, int p, int64_t &step) {
+  step = 0;
+  __int128 run = 1;
+  bool started = false;
+  for (int d = 0; d < g.ndim; ++d) {
+    const int64_t L = g.lengths[d];
+    if (L == 0) { step = 0; return true; }
+    if (L == 1) continue;
+    const int64_t S = g.strides[p][d];
+    if (!started) {
+      step = S;
+      run = L;
+      started = true;
+    } else {
+      if (run * (__int128)step != (__int128)S) return false;
+      run *= L;
     }
   }
   return true;
@@ -340,6 +403,16 @@ qs_status plan_setup(const qs_contract_desc &d, qs_plan *p) {
   const bool va = p->a_kfast ? pairs_adjacent(kg, 0, K) : pairs_adjacent(mg, 0, M);
   const bool vb = p->b_kfast ? pairs_adjacent(kg, 1, K) : pairs_adjacent(ng, 0, N);
   p->vec = (va && vb) ? 1 : 0;
+  // Single-ramp maps, in the device enumeration order.
+  {
+    const qs_group *grp[6] = {&mg, &kg, &ng, &kg, &mg, &ng};
+    const int map[6] = {0, 0, 0, 1, 1, 1};
+    const int bit[6] = {RAMP_MA, RAMP_KA, RAMP_NB, RAMP_KB, RAMP_MC, RAMP_NC};
+    p->ramps = 0;
+    for (int i = 0; i < 6; ++i)
+      if (map_ramp_step(*grp[i], map[i], p->steps[i])) p->ramps |= bit[i];
+      else p->steps[i] = 0;
+  }
   p->tile = TILE_64x64;
   p->k_splits = 1;
   p->k_chunk = K;
@@ -583,6 +656,16 @@ GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *
   g.b_bytes = buf || g.dom_buf ? (uint32_t)b_bytes : 0;
   g.batch_off = nullptr;
   g.batch = 1;
+  g.xcds = device_facts().xcds;
+  g.ramps = p->ramps;
+  g.step_mA = p->steps[0];
+  g.step_kA = p->steps[1];
+  g.step_nB = p->steps[2];
+  g.step_kB = p->steps[3];
+  g.step_mC = p->steps[4];
+  g.step_nC = p->steps[5];
+  g.kpad_a = k_pad_offset(p, 0);
+  g.kpad_b = k_pad_offset(p, 1);
   return g;
 }
 
@@ -606,6 +689,7 @@ qs_status run_device(qs_plan *p, const void *A, int64_t a_base, const void *B, i
   } else if (p->path == QS_PATH_OUTER) {
     e = launch_outer(g, s);
   } else if (p->path == QS_PATH_SPLIT) {
+    if (p->split_report_on == 2) return run_split_guarded(p, g, s);
     e = launch_split(p, g, p->split_report_on != 0, s);
   } else {
     e = launch_gemm(g, s);
@@ -643,7 +727,7 @@ qs_status qs_target_profile_get(qs_target_profile *out) {
   out->memory_bus_bits = pr.memoryBusWidth;
   out->l2_bytes = pr.l2CacheSize;
   out->hbm_bytes = (int64_t)pr.totalGlobalMem;
-  out->xcds = 8;
+  out->xcds = device_facts().xcds;
   out->mfma_f64_m = 16; out->mfma_f64_n = 16; out->mfma_f64_k = 4;
   out->mfma_f32_m = 16; out->mfma_f32_n = 16; out->mfma_f32_k = 4;
   out->mfma_i8_m = 16; out->mfma_i8_n = 16; out->mfma_i8_k = 64;
@@ -671,8 +755,10 @@ qs_status qs_plan_release(qs_plan *p) {
   if (p->released) return QS_OK;
   // Work already queued on the workspace finishes before it is handed back.
   if (hipDeviceSynchronize() != hipSuccess) return fail(QS_ERR_DEVICE, "release!: device sync");
+  ws_free(p, p->d_cbackup, (size_t)p->cbackup_bytes);
   ws_free(p, p->d_split, (size_t)p->split_bytes);
   ws_free(p, p->d_partials, (size_t)p->partials_bytes);
+  p->d_cbackup = nullptr;
   p->d_split = nullptr;
   p->d_partials = nullptr;
   p->released = 1;
@@ -731,9 +817,10 @@ qs_status qs_plan_rebind(qs_plan *p, const void *a, int64_t a_base, int64_t a_le
 
 qs_status qs_plan_destroy(qs_plan *p) {
   if (!p) return QS_OK;
-  if (!p->released && (p->d_split || p->d_partials)) {
+  if (!p->released && (p->d_split || p->d_partials || p->d_cbackup)) {
     // Queued work may still read the workspace; an allocator may hand it on at once.
     if (p->has_al) (void)hipDeviceSynchronize();
+    ws_free(p, p->d_cbackup, (size_t)p->cbackup_bytes);
     ws_free(p, p->d_split, (size_t)p->split_bytes);
     ws_free(p, p->d_partials, (size_t)p->partials_bytes);
   }
@@ -749,7 +836,8 @@ qs_status qs_plan_destroy(qs_plan *p) {
 
 qs_status qs_plan_set_error_report(qs_plan *p, int32_t on) {
   if (!p) return fail(QS_ERR_ARGUMENT, "qs_plan_set_error_report: null plan");
-  p->split_report_on = on ? 1 : 0;
+  if (on < 0 || on > 2) return fail(QS_ERR_ARGUMENT, "qs_plan_set_error_report: mode must be 0, 1 or 2");
+  p->split_report_on = on;
   return QS_OK;
 }
 
@@ -760,6 +848,9 @@ qs_status qs_plan_get_error_report(const qs_plan *p, qs_error_report *out) {
   out->samples = p->split_report.samples;
   out->rel_fro = p->split_report.rel_fro;
   out->max_rel = p->split_report.max_rel;
+  out->exceeded = p->split_report.exceeded;
+  out->escalations = p->split_report.escalations;
+  out->fallback = p->split_report.fallback;
   out->target = p->desc.target_rel_err;
   return QS_OK;
 }

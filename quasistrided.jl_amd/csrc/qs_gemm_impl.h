@@ -24,6 +24,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "qs_internal.h"
@@ -223,11 +224,66 @@ struct LdsTile {
 
 
 // XCD-aware workgroup order: the hardware deals workgroups round-robin over
-// the 8 XCDs, so consecutive ids land on different L2s.  Renumber so each XCD
+// the X XCDs, so consecutive ids land on different L2s.  Renumber so each XCD
 // walks a contiguous run of tiles (which share B columns and A rows).
-__device__ __forceinline__ int64_t xcd_swizzle(int64_t bid, int64_t nwg) {
-  if (nwg % 8 != 0) return bid;
-  return (bid % 8) * (nwg / 8) + bid / 8;
+__device__ __forceinline__ int64_t xcd_swizzle(int64_t bid, int64_t nwg, int X) {
+  if (X <= 1 || nwg % X != 0) return bid;
+  return (bid % X) * (nwg / X) + bid / X;
+}
+
+// The tile (tm, tn) of workgroup `bid` of a one-contraction launch of `nwg`
+// workgroups (GemmArgs::map_*): XCD x = bid % X takes patch x of the tile
+// grid, and its j-th workgroup the j-th tile of that patch, walked in groups
+// of map_gh tile rows along N.  Without a patch map, column-major order
+// through the contiguous-run renumbering.
+__device__ __forceinline__ void tile_of(const GemmArgs &g, int64_t bid, int64_t nwg, int64_t tiles_m, int64_t &tm,
+                                        int64_t &tn) {
+  const int X = g.xcds;
+  if (g.map_gx > 0 && X > 1 && nwg % X == 0) {
+    const int64_t x = bid % X, j = bid / X;
+    const int64_t pm = g.map_pm, pn = g.map_pn, gh = g.map_gh;
+    const int64_t grp = j / (gh * pn), r = j % (gh * pn);
+    const int64_t h = min(gh, pm - grp * gh);
+    tm = (x % g.map_gx) * pm + grp * gh + r % h;
+    tn = (x / g.map_gx) * pn + r / h;
+  } else {
+    const int64_t v = xcd_swizzle(bid, nwg, X);
+    tm = v % tiles_m;
+    tn = v / tiles_m;
+  }
+}
+
+// Host: the patch map of a BM x BN launch over tiles_m x tiles_n tiles.  The
+// grid is cut into X = gx * gy patches (gx along M) that divide it exactly;
+// among those, the one whose patch needs the fewest A rows plus B columns
+// (what each XCD's L2 must fetch per K step).  Inside a patch, groups of gh
+// tile rows make the workgroups one XCD runs at once (two per CU) about
+// square in rows and columns.  No exact cut: map_gx stays 0.
+inline void set_tile_map(GemmArgs &g, int bm, int bn, int64_t tiles_m, int64_t tiles_n, int cus) {
+  g.map_gx = g.map_pm = g.map_pn = g.map_gh = 0;
+  const int X = g.xcds;
+  if (X <= 1 || (tiles_m * tiles_n) % X != 0) return;
+  int64_t best = -1;
+  for (int gx = 1; gx <= X; ++gx) {
+    if (X % gx != 0 || tiles_m % gx != 0 || tiles_n % (X / gx) != 0) continue;
+    const int64_t pm = tiles_m / gx, pn = tiles_n / (X / gx);
+    const int64_t cost = pm * bm + pn * bn;
+    if (best < 0 || cost < best) {
+      best = cost;
+      g.map_gx = gx;
+      g.map_pm = (int)pm;
+      g.map_pn = (int)pn;
+    }
+  }
+  if (g.map_gx == 0) return;
+  // Concurrent workgroups per XCD (two per CU): gh rows x (conc / gh) columns
+  // of tiles with gh * bm close to (conc / gh) * bn.
+  const double conc = 2.0 * cus / X;
+  const int target = std::max(1, (int)(std::sqrt(conc * bn / bm) + 0.5));
+  int gh = 1;
+  for (int d = 1; d <= g.map_pm; ++d)
+    if (g.map_pm % d == 0 && d <= target) gh = d;
+  g.map_gh = gh;
 }
 
 typedef const __attribute__((address_space(4))) int64_t *qs_const_i64;
@@ -289,8 +345,10 @@ struct Gather {
 
   // `lo` / `bytes`: the operand's accessed span (elements from its storage
   // start, and its length in bytes) for the buffer loads.
+  // rramp: the row map is a single ramp of step rstep (no table read).
   __device__ __forceinline__ void init(int t, int64_t r0, int64_t rows_total, int64_t base,
-                                       const int64_t *__restrict__ rowtab, const void *p, int64_t lo, uint32_t bytes) {
+                                       const int64_t *__restrict__ rowtab, bool rramp, int64_t rstep, const void *p,
+                                       int64_t lo, uint32_t bytes) {
     row0 = KF ? t / KU : VW * (t % RU);
     kk0 = KF ? VW * (t % KU) : t / RU;
     if (UNIFORM_K) kk0 = __builtin_amdgcn_readfirstlane(kk0);
@@ -301,7 +359,8 @@ struct Gather {
       const int64_t gi = r0 + row0 + r * RSTEP;
       rowok[r] = gi < rows_total && (!PARTIAL || t + 256 * r < ROWS * BK);
       // Past the last row: the last one (the last pair; rows_total is even when VW == 2).
-      const int64_t o = base + rowtab[gi < rows_total ? gi : rows_total - (KF ? 1 : VW)];
+      const int64_t gc = gi < rows_total ? gi : rows_total - (KF ? 1 : VW);
+      const int64_t o = base + (rramp ? gc * rstep : rowtab[gc]);
       // Row and K byte offsets add up, modulo 2^32, to the element's offset in the span.
       rowoff[r] = BUF ? (Off)(uint32_t)((o - lo) * ESZ) : (Off)o;
     }
@@ -320,6 +379,27 @@ struct Gather {
         kok[S % KS][r] = gk < kend;
         const int64_t gkc = gk < kend ? gk : kend - 1;
         koff[S % KS][r] = UNIFORM_K ? ((qs_const_i64)ktab)[gkc] : ktab[gkc];
+      }
+    }
+  }
+  // fetch_k for a stage on a tile's critical path (the first ones): with a
+  // single-ramp K map (kramp) the offsets are computed, not read; K steps
+  // past kend take the tables' pad value.
+  template <int S>
+  __device__ __forceinline__ void fetch_k_first(int64_t k0, int64_t kend, const int64_t *__restrict__ ktab, bool kramp,
+                                                int64_t kstep, int64_t kpad) {
+    if (!kramp) {
+      fetch_k<S>(k0, kend, ktab);
+      return;
+    }
+#pragma unroll
+    for (int r = 0; r < NK; ++r) {
+      const int64_t gk = k0 + kk0 + r * KSTEP;
+      if constexpr (BUF) {
+        koff[S % KS][r] = (Off)(uint32_t)(gk < kend ? gk * kstep : kpad);
+      } else {
+        kok[S % KS][r] = gk < kend;
+        koff[S % KS][r] = (gk < kend ? gk : kend - 1) * kstep;
       }
     }
   }
@@ -483,8 +563,8 @@ struct TileGemm {
         static_assert(k_table_reach(BK, NS) <= kKTablePad, "K table pad shorter than the gather's reach");
         Gather<R, CA, MIXED, BM, BK, AKF, NS, VEC ? 2 : 1> ga;
         Gather<R, CB, MIXED, BN, BK, BKF, NS, VEC ? 2 : 1, BAUX> gb;
-        ga.init(t, m0, g.M, g.a_base, g.mA, g.A, g.a_lo, g.a_bytes);
-        gb.init(t, n0, g.N, g.b_base, g.nB, g.B, g.b_lo, g.b_bytes);
+        ga.init(t, m0, g.M, g.a_base, g.mA, g.ramps & RAMP_MA, g.step_mA, g.A, g.a_lo, g.a_bytes);
+        gb.init(t, n0, g.N, g.b_base, g.nB, g.ramps & RAMP_NB, g.step_nB, g.B, g.b_lo, g.b_bytes);
         typedef decltype(ga) GA;
         typedef decltype(gb) GB;
         auto step = [&](int64_t k0, auto u) {
@@ -541,8 +621,9 @@ struct TileGemm {
           // The same issue order as the loop ([per-lane offsets of stage
           // s + NS - 1][loads of stage s]), so the wait counts agree on entry
           // and on the back edge.
-          ga.template fetch_k<0>(kbeg, kend, g.kA);
-          gb.template fetch_k<0>(kbeg, kend, g.kB);
+          const bool kra = g.ramps & RAMP_KA, krb = g.ramps & RAMP_KB;
+          ga.template fetch_k_first<0>(kbeg, kend, g.kA, kra, g.step_kA, g.kpad_a);
+          gb.template fetch_k_first<0>(kbeg, kend, g.kB, krb, g.step_kB, g.kpad_b);
           if constexpr (NS == 4) {
             if constexpr (!GA::UNIFORM_K) {
               ga.template fetch_k<1>(kbeg + BK, kend, g.kA);
@@ -555,13 +636,15 @@ struct TileGemm {
           }
           auto pro = [&](auto sc) {
             constexpr int S = decltype(sc)::value;
-            if constexpr (!GA::UNIFORM_K) ga.template fetch_k<(S + NS - 1) % NS>(kbeg + (S + NS - 1) * BK, kend, g.kA);
-            if constexpr (!GB::UNIFORM_K) gb.template fetch_k<(S + NS - 1) % NS>(kbeg + (S + NS - 1) * BK, kend, g.kB);
+            if constexpr (!GA::UNIFORM_K)
+              ga.template fetch_k_first<(S + NS - 1) % NS>(kbeg + (S + NS - 1) * BK, kend, g.kA, kra, g.step_kA, g.kpad_a);
+            if constexpr (!GB::UNIFORM_K)
+              gb.template fetch_k_first<(S + NS - 1) % NS>(kbeg + (S + NS - 1) * BK, kend, g.kB, krb, g.step_kB, g.kpad_b);
             __builtin_amdgcn_sched_barrier(0);
             ga.template load<S>(g.A, g.a_type);
             gb.template load<S>(g.B, g.b_type);
-            if constexpr (GA::UNIFORM_K) ga.template fetch_k<0>(kbeg + (S + 1) * BK, kend, g.kA);
-            if constexpr (GB::UNIFORM_K) gb.template fetch_k<0>(kbeg + (S + 1) * BK, kend, g.kB);
+            if constexpr (GA::UNIFORM_K) ga.template fetch_k_first<0>(kbeg + (S + 1) * BK, kend, g.kA, kra, g.step_kA, g.kpad_a);
+            if constexpr (GB::UNIFORM_K) gb.template fetch_k_first<0>(kbeg + (S + 1) * BK, kend, g.kB, krb, g.step_kB, g.kpad_b);
             __builtin_amdgcn_sched_barrier(0);
           };
           pro(std::integral_constant<int, 0>());
@@ -593,16 +676,17 @@ struct TileGemm {
       static_assert(k_table_reach(BK, 1) <= kKTablePad, "K table pad shorter than the gather's reach");
       Gather<R, CA, MIXED, BM, BK, AKF> ga;
       Gather<R, CB, MIXED, BN, BK, BKF> gb;
-      ga.init(t, m0, g.M, g.a_base, g.mA, g.A, g.a_lo, g.a_bytes);
-      gb.init(t, n0, g.N, g.b_base, g.nB, g.B, g.b_lo, g.b_bytes);
+      ga.init(t, m0, g.M, g.a_base, g.mA, g.ramps & RAMP_MA, g.step_mA, g.A, g.a_lo, g.a_bytes);
+      gb.init(t, n0, g.N, g.b_base, g.nB, g.ramps & RAMP_NB, g.step_nB, g.B, g.b_lo, g.b_bytes);
       int buf = 0;
       if (kbeg < kend) {
-        ga.template fetch_k<0>(kbeg, kend, g.kA);
-        gb.template fetch_k<0>(kbeg, kend, g.kB);
+        const bool kra = g.ramps & RAMP_KA, krb = g.ramps & RAMP_KB;
+        ga.template fetch_k_first<0>(kbeg, kend, g.kA, kra, g.step_kA, g.kpad_a);
+        gb.template fetch_k_first<0>(kbeg, kend, g.kB, krb, g.step_kB, g.kpad_b);
         ga.template load<0>(g.A, g.a_type);
         gb.template load<0>(g.B, g.b_type);
-        ga.template fetch_k<0>(kbeg + BK, kend, g.kA);
-        gb.template fetch_k<0>(kbeg + BK, kend, g.kB);
+        ga.template fetch_k_first<0>(kbeg + BK, kend, g.kA, kra, g.step_kA, g.kpad_a);
+        gb.template fetch_k_first<0>(kbeg + BK, kend, g.kB, krb, g.step_kB, g.kpad_b);
         ga.template stage<0, LA::plane>(smem, t, sa);
         gb.template stage<0, LB::plane>(smem + SA, t, sb);
       }
@@ -691,13 +775,13 @@ struct TileGemm {
         for (int r = 0; r < 4; ++r) {
           const int64_t row = m0 + wm * WTM + a * 16 + mfma_row<R>(fk, r);
           rok[r] = row < g.M;
-          moff[r] = rok[r] ? g.mC[row] : 0;
+          moff[r] = !rok[r] ? 0 : (g.ramps & RAMP_MC) ? row * g.step_mC : g.mC[row];
         }
 #pragma unroll
         for (int b = 0; b < FN; ++b) {
           const int64_t col = n0 + wn * WTN + b * 16 + fr;
           cok[b] = col < g.N;
-          noff[b] = cok[b] ? g.nC[col] : 0;
+          noff[b] = !cok[b] ? 0 : (g.ramps & RAMP_NC) ? col * g.step_nC : g.nC[col];
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -721,13 +805,13 @@ struct TileGemm {
         for (int r = 0; r < 4; ++r) {
           const int64_t row = m0 + wm * WTM + a * 16 + mfma_row<R>(fk, r);
           rok[r] = row < g.M;
-          moff[r] = rok[r] ? g.mC[row] : 0;
+          moff[r] = !rok[r] ? 0 : (g.ramps & RAMP_MC) ? row * g.step_mC : g.mC[row];
         }
 #pragma unroll
         for (int b = 0; b < FN; ++b) {
           const int64_t col = n0 + wn * WTN + b * 16 + fr;
           cok[b] = col < g.N;
-          noff[b] = cok[b] ? g.nC[col] : 0;
+          noff[b] = !cok[b] ? 0 : (g.ramps & RAMP_NC) ? col * g.step_nC : g.nC[col];
         }
         R o[4][FN], oi[4][FN];
 #pragma unroll
@@ -813,9 +897,9 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmArgs g) {
   __shared__ R smem[TG::SMEM];
   const int64_t tiles_m = (g.M + BM - 1) / BM;
   const int64_t tiles_n = (g.N + BN - 1) / BN;
-  const int64_t bid = xcd_swizzle(blockIdx.x, (int64_t)gridDim.x);
-  const int64_t tm = bid % tiles_m, tn = bid / tiles_m;
-  if (tn >= tiles_n) return;
+  int64_t tm, tn;
+  tile_of(g, blockIdx.x, (int64_t)gridDim.x, tiles_m, tm, tn);
+  if (tn >= tiles_n || tm >= tiles_m) return;
   TG::template run<true>(g, tm, tn, blockIdx.y, smem);
 }
 
@@ -834,7 +918,8 @@ __global__ __launch_bounds__(256, 2) void gemm_batched_kernel(const GemmArgs g) 
   const int64_t tiles_m = (g.M + BM - 1) / BM;
   const int64_t tiles_n = (g.N + BN - 1) / BN;
   const int64_t tiles = gridDim.x;
-  const int64_t lin = tiles > 1 ? xcd_swizzle((int64_t)blockIdx.z * tiles + blockIdx.x, tiles * (int64_t)gridDim.z)
+  const int64_t lin = tiles > 1 ? xcd_swizzle((int64_t)blockIdx.z * tiles + blockIdx.x, tiles * (int64_t)gridDim.z,
+                                             g.xcds)
                                  : (int64_t)blockIdx.z;
   const int64_t bid = lin % tiles, z = lin / tiles;
   const int64_t tm = bid % tiles_m, tn = bid / tiles_m;
@@ -863,6 +948,9 @@ static void launch_dom(const GemmArgs &g, dim3 grid, hipStream_t s) {
   else
     hipLaunchKernelGGL((gemm_kernel<R, true, BM, BN, false, false, MIXED, false, DOM>), grid, block, 0, s, g);
 }
+
+template <typename R, bool CPLX, int BM, int BN, bool MIXED>
+static hipError_t launch_tile_grid(const GemmArgs &g, dim3 grid, hipStream_t s);
 
 template <typename R, bool CPLX, int BM, int BN, bool MIXED>
 static hipError_t launch_tile_mixed(const GemmArgs &g, hipStream_t s) {
@@ -905,6 +993,15 @@ static hipError_t launch_tile_mixed(const GemmArgs &g, hipStream_t s) {
     }
   }
   dim3 grid((unsigned)tiles, (unsigned)g.k_splits), block(256);
+  GemmArgs gm = g;
+  set_tile_map(gm, BM, BN, (g.M + BM - 1) / BM, (g.N + BN - 1) / BN, device_facts().cus);
+  return launch_tile_grid<R, CPLX, BM, BN, MIXED>(gm, grid, s);
+}
+
+template <typename R, bool CPLX, int BM, int BN, bool MIXED>
+static hipError_t launch_tile_grid(const GemmArgs &g, dim3 grid, hipStream_t s) {
+  const int64_t tiles = grid.x;
+  const dim3 block(256);
   // Paired loads: real, one storage type, tiles on the two-stage pipeline.
   if constexpr (!MIXED && !CPLX && BM * BN <= 64 * 128) {
     if (g.vec) {

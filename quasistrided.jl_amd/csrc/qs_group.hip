@@ -47,11 +47,19 @@ struct GroupSmem {
 };
 
 // coef: per item {alpha_re, alpha_im, beta_re, beta_im}, indexed by item.index.
+// cycles (the autotuner's measurement, null otherwise): per member, the
+// workgroup time its tiles took, in ticks of the constant 100 MHz clock.
 template <typename R, bool CPLX, bool AKF, bool BKF>
 __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__restrict__ items, int n_items,
-                                                          int64_t total_tiles, const double *__restrict__ coef) {
+                                                          int64_t total_tiles, const double *__restrict__ coef,
+                                                          unsigned long long *cycles) {
   __shared__ R smem[GroupSmem<R, CPLX, AKF, BKF>::value];
   for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+    uint64_t t0 = 0;
+    if (cycles) {
+      __syncthreads();
+      t0 = wall_clock64();
+    }
     int lo = 0, hi = n_items - 1;
     while (lo < hi) {   // last item with tile_start <= t
       const int mid = (lo + hi + 1) >> 1;
@@ -73,34 +81,38 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
       case TILE_32x64: TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::template run<false>(g, tm, tn, 0, smem); break;
       default: TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::template run<false>(g, tm, tn, 0, smem); break;
     }
+    if (cycles) {
+      __syncthreads();
+      if (threadIdx.x == 0) atomicAdd(cycles + items[lo].index, (unsigned long long)(wall_clock64() - t0));
+    }
   }
 }
 
 template <typename R, bool CPLX, bool AKF, bool BKF>
 static hipError_t launch_grouped_t(const GroupItem *items, int n, int64_t total, const double *coef, int grid,
-                                   hipStream_t s) {
-  hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF>), dim3(grid), dim3(256), 0, s, items, n, total, coef);
+                                   hipStream_t s, unsigned long long *cycles) {
+  hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF>), dim3(grid), dim3(256), 0, s, items, n, total, coef, cycles);
   return hipGetLastError();
 }
 
 template <typename R, bool CPLX>
 static hipError_t launch_grouped_rc(int akf, int bkf, const GroupItem *items, int n, int64_t total,
-                                    const double *coef, int grid, hipStream_t s) {
+                                    const double *coef, int grid, hipStream_t s, unsigned long long *cy) {
   switch ((akf ? 2 : 0) | (bkf ? 1 : 0)) {
-    case 0: return launch_grouped_t<R, CPLX, false, false>(items, n, total, coef, grid, s);
-    case 1: return launch_grouped_t<R, CPLX, false, true>(items, n, total, coef, grid, s);
-    case 2: return launch_grouped_t<R, CPLX, true, false>(items, n, total, coef, grid, s);
-    default: return launch_grouped_t<R, CPLX, true, true>(items, n, total, coef, grid, s);
+    case 0: return launch_grouped_t<R, CPLX, false, false>(items, n, total, coef, grid, s, cy);
+    case 1: return launch_grouped_t<R, CPLX, false, true>(items, n, total, coef, grid, s, cy);
+    case 2: return launch_grouped_t<R, CPLX, true, false>(items, n, total, coef, grid, s, cy);
+    default: return launch_grouped_t<R, CPLX, true, true>(items, n, total, coef, grid, s, cy);
   }
 }
 
 static hipError_t launch_grouped(int ct, int akf, int bkf, const GroupItem *items, int n, int64_t total,
-                                 const double *coef, int grid, hipStream_t s) {
+                                 const double *coef, int grid, hipStream_t s, unsigned long long *cy = nullptr) {
   switch (ct) {
-    case QS_F64: return launch_grouped_rc<double, false>(akf, bkf, items, n, total, coef, grid, s);
-    case QS_F32: return launch_grouped_rc<float, false>(akf, bkf, items, n, total, coef, grid, s);
-    case QS_C64: return launch_grouped_rc<double, true>(akf, bkf, items, n, total, coef, grid, s);
-    default: return launch_grouped_rc<float, true>(akf, bkf, items, n, total, coef, grid, s);
+    case QS_F64: return launch_grouped_rc<double, false>(akf, bkf, items, n, total, coef, grid, s, cy);
+    case QS_F32: return launch_grouped_rc<float, false>(akf, bkf, items, n, total, coef, grid, s, cy);
+    case QS_C64: return launch_grouped_rc<double, true>(akf, bkf, items, n, total, coef, grid, s, cy);
+    default: return launch_grouped_rc<float, true>(akf, bkf, items, n, total, coef, grid, s, cy);
   }
 }
 
@@ -168,16 +180,8 @@ struct qs_graph {
 
 namespace qs {
 
-static int cu_count() {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    hipDeviceProp_t pr;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess) cus = pr.multiProcessorCount;
-    if (cus <= 0) cus = 256;
-  }
-  return cus;
-}
+// Two workgroups per CU: the persistent grid's width.
+static int cu_count() { return device_facts().cus; }
 
 static void free_launches(qs_group_plan *g) {
   for (auto &l : g->launches)
@@ -202,34 +206,41 @@ static qs_group_plan::Storages storages_of(const qs_plan *p) {
   return qs_group_plan::Storages{d.a, d.b, d.c, d.a_base, d.a_len, d.b_base, d.b_len, d.c_base, d.c_len};
 }
 
-// (Re)build the device tables from the plans and their tile classes.
-static qs_status build_tables(qs_group_plan *g) {
-  free_launches(g);
-  g->held.clear();
-  for (const qs_plan *p : g->plans) g->held.push_back(storages_of(p));
-  std::map<std::tuple<int, int, int, int>, std::vector<GroupItem>> classes;
+// The tile class a member takes: the caller's override, else the autotune
+// cache's choice for its grouped shape class, else the padding heuristic.
+static int member_class(const qs_group_plan *g, size_t i) {
+  const qs_plan *p = g->plans[i];
+  int cls = g->tile_override[i], splits = 1;
+  if (cls < 0 && !tune_find(group_shape_key(p), cls, splits)) cls = -1;
+  if (cls < 0) cls = grouped_tile(p->M, p->N, p->desc.compute_type);
+  return (cls == TILE_128x64 || cls == TILE_32x64) ? cls : TILE_64x64;
+}
+
+// Device tables of grouped launches for the members with cls[i] >= 0 (one
+// launch per compute type and gather directions).  scratch != null: every
+// member writes C into `scratch` instead (the autotuner's runs).
+static qs_status make_launches(qs_group_plan *g, const std::vector<int> &cls, void *scratch, std::vector<Launch> &out) {
+  std::map<std::tuple<int, int, int>, std::vector<GroupItem>> by_launch;
   for (size_t i = 0; i < g->plans.size(); ++i) {
+    if (cls[i] < 0) continue;
     qs_plan *p = g->plans[i];
-    GemmArgs a = make_args(p, p->desc.a, base_a(g, i), p->desc.b, base_b(g, i), p->desc.c, base_c(g, i));
+    GemmArgs a = scratch ? make_args(p, p->desc.a, base_a(g, i), p->desc.b, base_b(g, i), scratch,
+                                     -(p->mC_lo + p->nC_lo))
+                         : make_args(p, p->desc.a, base_a(g, i), p->desc.b, base_b(g, i), p->desc.c, base_c(g, i));
     a.k_splits = 1;
     a.k_chunk = p->K;
     a.partials = nullptr;
     GroupItem it;
     std::memset(&it, 0, sizeof it);
     it.g = a;
-    if (a.mixed) {   // the generic gather (mixed types, spans past 2 GiB) runs standalone, on the same stream
-      g->standalone.push_back((int)i);
-      continue;
-    }
-    it.cls = g->tile_override[i] >= 0 ? g->tile_override[i] : grouped_tile(p->M, p->N, p->desc.compute_type);
-    if (it.cls != TILE_128x64 && it.cls != TILE_32x64) it.cls = TILE_64x64;
+    it.cls = cls[i];
     const TileShape ts = tile_shape(it.cls, p->desc.compute_type, p->a_kfast, p->b_kfast);
     it.tiles_m = (p->M + ts.bm - 1) / ts.bm;
     it.tile_start = it.tiles_m * ((p->N + ts.bn - 1) / ts.bn);   // tile count, prefix-summed below
     it.index = (int32_t)i;
-    classes[std::make_tuple(p->desc.compute_type, p->a_kfast, p->b_kfast, 0)].push_back(it);
+    by_launch[std::make_tuple(p->desc.compute_type, p->a_kfast, p->b_kfast)].push_back(it);
   }
-  for (auto &kv : classes) {
+  for (auto &kv : by_launch) {
     std::vector<GroupItem> &v = kv.second;
     // Longest K first: a tile's cost grows with K, so the persistent grid's
     // round-robin hands out the long tiles first.
@@ -244,23 +255,44 @@ static qs_status build_tables(qs_group_plan *g) {
     l.ct = std::get<0>(kv.first);
     l.akf = std::get<1>(kv.first);
     l.bkf = std::get<2>(kv.first);
-    l.mixed = std::get<3>(kv.first);
+    l.mixed = 0;
     l.n = (int)v.size();
     l.total = start;
     l.grid = (int)std::min<int64_t>(start, 2 * (int64_t)cu_count());
     if (l.grid < 1) l.grid = 1;
-    if (dev_malloc(&l.d_items, sizeof(GroupItem) * v.size()) != hipSuccess) {
-      free_launches(g);
+    if (dev_malloc(&l.d_items, sizeof(GroupItem) * v.size()) != hipSuccess)
       return fail(QS_ERR_MEMORY, "qs_group_create: device table");
-    }
-    if (hipMemcpy(l.d_items, v.data(), sizeof(GroupItem) * v.size(), hipMemcpyHostToDevice) != hipSuccess) {
-      (void)dev_free(l.d_items);
-      free_launches(g);
+    out.push_back(l);
+    if (hipMemcpy(l.d_items, v.data(), sizeof(GroupItem) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
       return fail(QS_ERR_DEVICE, "qs_group_create: table upload");
-    }
-    g->launches.push_back(l);
   }
   return QS_OK;
+}
+
+static void free_launch_list(std::vector<Launch> &ls) {
+  for (auto &l : ls)
+    if (l.d_items) (void)dev_free(l.d_items);
+  ls.clear();
+}
+
+// (Re)build the device tables from the plans and their tile classes.
+static qs_status build_tables(qs_group_plan *g) {
+  free_launches(g);
+  g->held.clear();
+  for (const qs_plan *p : g->plans) g->held.push_back(storages_of(p));
+  std::vector<int> cls(g->plans.size(), -1);
+  for (size_t i = 0; i < g->plans.size(); ++i) {
+    qs_plan *p = g->plans[i];
+    GemmArgs a = make_args(p, p->desc.a, base_a(g, i), p->desc.b, base_b(g, i), p->desc.c, base_c(g, i));
+    if (a.mixed) {   // the generic gather (mixed types, spans past 2 GiB) runs standalone, on the same stream
+      g->standalone.push_back((int)i);
+      continue;
+    }
+    cls[i] = member_class(g, i);
+  }
+  qs_status st = make_launches(g, cls, nullptr, g->launches);
+  if (st != QS_OK) free_launches(g);
+  return st;
 }
 
 static qs_status set_coefficients(qs_group_plan *g, const double *alphas, const double *betas, hipStream_t s) {
@@ -532,6 +564,155 @@ qs_status qs_group_execute(qs_group_plan *g, const double *alphas, const double 
   st = set_coefficients(g, alphas, betas, s);
   if (st != QS_OK) return st;
   return launch_all(g, s);
+}
+
+// Per-shape-class tiles for grouped launches, timed in grouped context.  The
+// whole group runs (writing a scratch C, with alpha = 1 and beta = 0) once
+// with every tunable member on each grouped tile class, the kernel recording
+// the workgroup time each member's tiles take inside that mixed launch.  A
+// class's cost for a tile is the sum over its members (group_shape_key:
+// types, gather directions, octave buckets of M, N, K); the cheapest wins.
+// The tuned assignment is then timed against the padding heuristic's, whole
+// group against whole group: it is kept only if it is faster, so tuning
+// never slows the group down.  The result goes to the autotune cache (saved
+// by qs_autotune_save) and the group's tables are rebuilt.  Every tile sums
+// K in the same order, so results do not change.  Members with a tile
+// override or on the generic gather are left alone.
+qs_status qs_group_autotune(qs_group_plan *g, void *stream, int32_t reps, qs_group_tune_result *out) {
+  if (!g) return fail(QS_ERR_ARGUMENT, "qs_group_autotune: null group");
+  if (out) std::memset(out, 0, sizeof *out);
+  if (reps <= 0) reps = 3;
+  hipStream_t s = (hipStream_t)stream;
+  qs_status st = check_member_spans(g, "qs_group_autotune");
+  if (st != QS_OK) return st;
+  if ((st = refresh_tables(g)) != QS_OK) return st;
+  const size_t n = g->plans.size();
+  std::vector<int> fixed(n, -1), heur(n, -1), before_cls(n, -1);
+  std::map<std::string, std::vector<size_t>> classes;
+  size_t scratch_bytes = 16;
+  for (size_t i = 0; i < n; ++i) {
+    const qs_plan *p = g->plans[i];
+    GemmArgs a = make_args(p, p->desc.a, base_a(g, i), p->desc.b, base_b(g, i), p->desc.c, base_c(g, i));
+    if (a.mixed) continue;   // standalone members take no part
+    scratch_bytes = std::max(scratch_bytes, (size_t)(p->mC_hi + p->nC_hi - p->mC_lo - p->nC_lo + 1) *
+                                                (size_t)elem_bytes(p->desc.c_type));
+    before_cls[i] = member_class(g, i);
+    if (g->tile_override[i] >= 0) {
+      fixed[i] = before_cls[i];
+      continue;
+    }
+    const int h = grouped_tile(p->M, p->N, p->desc.compute_type);
+    heur[i] = (h == TILE_128x64 || h == TILE_32x64) ? h : TILE_64x64;
+    classes[group_shape_key(p)].push_back(i);
+  }
+  if (classes.empty()) return QS_OK;
+  void *scratch = nullptr;
+  double *d_coef = nullptr;
+  unsigned long long *d_cyc = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (dev_malloc(&scratch, scratch_bytes) != hipSuccess || dev_malloc(&d_coef, sizeof(double) * 4 * n) != hipSuccess ||
+      dev_malloc(&d_cyc, sizeof(unsigned long long) * n) != hipSuccess)
+    st = fail(QS_ERR_MEMORY, "qs_group_autotune: buffers");
+  if (st == QS_OK) {
+    std::vector<double> coef(4 * n, 0.0);
+    for (size_t j = 0; j < n; ++j) coef[4 * j] = 1.0;   // alpha = 1, beta = 0
+    if (hipMemcpy(d_coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice) != hipSuccess)
+      st = fail(QS_ERR_DEVICE, "qs_group_autotune: coefficient upload");
+  }
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  // One assignment of classes: a warm-up run, a run recording each member's
+  // tile time (cyc, when given), then `reps` timed runs (ms, when given).
+  auto run = [&](const std::vector<int> &cls, std::vector<unsigned long long> *cyc, double *ms) -> qs_status {
+    std::vector<Launch> ls;
+    qs_status r = make_launches(g, cls, scratch, ls);
+    hipError_t e = hipSuccess;
+    auto all = [&](unsigned long long *cy) {
+      for (const Launch &l : ls)
+        if (e == hipSuccess && l.total > 0) e = launch_grouped(l.ct, l.akf, l.bkf, l.d_items, l.n, l.total, d_coef, l.grid, s, cy);
+    };
+    if (r == QS_OK) all(nullptr);
+    if (r == QS_OK && cyc) {
+      if (e == hipSuccess) e = hipMemsetAsync(d_cyc, 0, sizeof(unsigned long long) * n, s);
+      all(d_cyc);
+      cyc->assign(n, 0);
+      if (e == hipSuccess) e = hipMemcpyAsync(cyc->data(), d_cyc, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    if (r == QS_OK && ms) {
+      (void)hipEventRecord(e0, s);
+      for (int k = 0; k < reps; ++k) all(nullptr);
+      (void)hipEventRecord(e1, s);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float f = 0;
+      (void)hipEventElapsedTime(&f, e0, e1);
+      *ms = (double)f / reps;
+    }
+    if (r == QS_OK && e != hipSuccess) r = fail(QS_ERR_DEVICE, std::string("qs_group_autotune: ") + hipGetErrorString(e));
+    if (hipStreamSynchronize(s) != hipSuccess && r == QS_OK) r = fail(QS_ERR_DEVICE, "qs_group_autotune: sync");
+    free_launch_list(ls);
+    return r;
+  };
+  const int cands[3] = {TILE_128x64, TILE_64x64, TILE_32x64};
+  std::vector<unsigned long long> cyc[3];
+  for (int c = 0; c < 3 && st == QS_OK; ++c) {
+    std::vector<int> cls = fixed;
+    for (auto &kv : classes)
+      for (size_t i : kv.second) cls[i] = cands[c];
+    st = run(cls, &cyc[c], nullptr);
+  }
+  std::vector<int> tuned = fixed, heuristic = fixed;
+  std::map<std::string, int> best_of;
+  for (auto &kv : classes) {
+    int best = 0;
+    unsigned long long best_cost = 0;
+    for (int c = 0; c < 3 && st == QS_OK; ++c) {
+      unsigned long long cost = 0;
+      for (size_t i : kv.second) cost += cyc[c][i];
+      if (c == 0 || cost < best_cost) { best = c; best_cost = cost; }
+    }
+    best_of[kv.first] = cands[best];
+    for (size_t i : kv.second) {
+      tuned[i] = cands[best];
+      heuristic[i] = heur[i];
+    }
+  }
+  // Whole group against whole group, alternating.
+  double t_tuned = 1e300, t_heur = 1e300;
+  for (int k = 0; k < 2 && st == QS_OK; ++k) {
+    double a = 0, b = 0;
+    st = run(tuned, nullptr, &a);
+    if (st == QS_OK) st = run(heuristic, nullptr, &b);
+    t_tuned = std::min(t_tuned, a);
+    t_heur = std::min(t_heur, b);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)dev_free(scratch);
+  (void)dev_free(d_coef);
+  (void)dev_free(d_cyc);
+  if (st != QS_OK) return st;
+  const bool keep = t_tuned < t_heur;
+  for (auto &kv : classes)
+    tune_store(kv.first, keep ? best_of[kv.first] : heur[kv.second[0]], 1, keep ? t_tuned : t_heur);
+  if (!keep) {
+    // The heuristic's per-member tiles: a class may hold members the
+    // heuristic treats differently, so its cache entry is dropped.
+    for (auto &kv : classes) tune_erase(kv.first);
+  }
+  st = build_tables(g);
+  if (st != QS_OK) return st;
+  int changed = 0;
+  for (size_t i = 0; i < n; ++i)
+    if (before_cls[i] >= 0 && member_class(g, i) != before_cls[i]) ++changed;
+  if (out) {
+    out->classes = (int32_t)classes.size();
+    out->candidates = 3;
+    out->members_changed = changed;
+    out->before_ms = t_heur;
+    out->after_ms = keep ? t_tuned : t_heur;
+  }
+  return QS_OK;
 }
 
 qs_status qs_group_flops(const qs_group_plan *g, int64_t *flops) {

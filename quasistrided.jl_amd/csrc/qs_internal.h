@@ -84,7 +84,38 @@ struct GemmArgs {
   // operand in its real type, spans below 2 GiB: both gathers take buffer
   // loads (a_lo / a_bytes and b_lo / b_bytes are set).
   int dom_buf = 0;
+  // Workgroup -> tile order of a one-contraction launch.  The hardware deals
+  // workgroups round-robin over `xcds` XCDs (one L2 each).  map_gx > 0: the
+  // tile grid is cut into xcds patches of map_pm x map_pn tiles (map_gx of
+  // them along M), one per XCD, so each L2 holds a slice of A's rows and of
+  // B's columns; inside a patch, groups of map_gh tile rows are walked along
+  // N.  map_gx == 0: column-major order, renumbered so each XCD walks a
+  // contiguous run.
+  int xcds = 8;
+  int map_gx = 0, map_pm = 0, map_pn = 0, map_gh = 0;
+  // Maps that are a single ramp (offset of coordinate q = q * step,
+  // affine_ramp of src/layout/axis_group.jl:149-180): the kernels compute
+  // those offsets instead of reading the table where the read would sit on
+  // a tile's critical path (row offsets, the first stages' K offsets, C's
+  // offsets in the epilogue).  Bits RAMP_*; kpad_*: the K tables' pad value.
+  int ramps = 0;
+  int64_t step_mA = 0, step_kA = 0, step_nB = 0, step_kB = 0, step_mC = 0, step_nC = 0;
+  int64_t kpad_a = 0, kpad_b = 0;
 };
+enum RampBits { RAMP_MA = 1, RAMP_KA = 2, RAMP_NB = 4, RAMP_KB = 8, RAMP_MC = 16, RAMP_NC = 32 };
+
+// Facts of the current device that planning and launches use
+// (hipGetDeviceProperties, cached per device): the gfx950 TargetProfile's
+// counterpart of src/hardware/target.jl:26-47.
+struct DeviceFacts {
+  int cus = 256;          // compute units
+  int xcds = 8;           // XCDs (one L2 each); workgroups are dealt round-robin over them
+  int wave = 64;          // wavefront size
+  int lds_per_cu = 160 * 1024;
+  int lds_per_block = 64 * 1024;
+  char arch[64] = {0};
+};
+const DeviceFacts &device_facts();
 
 // Launch the tiled MFMA GEMM (with k_splits > 1, a second kernel reduces the
 // partials in a fixed order).  Shapes and tables must already be validated.

@@ -2,6 +2,7 @@
 // src/planning/plan.jl:16-45, and its ContractWorkspace, workspace.jl:24-52).
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "qs_internal.h"
@@ -18,6 +19,8 @@ struct qs_plan {
   int c_mfast = 1;             // C's unit-stride side: M (1) or N (0)
   int mixed = 0;
   int vec = 0;                // the gathers' element pairs are adjacent (GemmArgs::vec)
+  int ramps = 0;              // RampBits of the maps that are single ramps, and their steps
+  int64_t steps[6] = {0, 0, 0, 0, 0, 0};   // mA kA nB kB mC nC
   int moduli = 0;             // split precision: CRT moduli (INT8 GEMMs)
   int64_t partials_bytes = 0;   // capacity of d_partials (split-K partial sums)
   // Device offset tables: mA mC nB nC kA kB, one allocation.
@@ -41,8 +44,17 @@ struct qs_plan {
   // priority), forked from and joined back to the caller's stream by events.
   hipStream_t split_side = nullptr;
   hipEvent_t split_fork = nullptr, split_join = nullptr;
+  // 0: no report; 1: report; 2: guarded -- an execute whose report misses
+  // the target is redone with more moduli, then with the plain FP64 kernel.
   int split_report_on = 0;
-  struct { int32_t samples; double rel_fro, max_rel; } split_report = {0, 0.0, 0.0};
+  struct {
+    int32_t samples;
+    double rel_fro, max_rel;
+    int32_t exceeded, escalations, fallback;
+  } split_report = {0, 0.0, 0.0, 0, 0, 0};
+  // Guarded mode with beta != 0: C's span as it was before the execute.
+  void *d_cbackup = nullptr;
+  int64_t cbackup_bytes = 0;
   // Host-memory staging buffers (A, B, C spans) and their capacities in bytes.
   void *stage[3] = {nullptr, nullptr, nullptr};
   size_t stage_cap[3] = {0, 0, 0};
@@ -91,12 +103,19 @@ qs_status configure_gemm(qs_plan *p, int tile, int splits);
 // The autotune cache's choice for the plan's shape class, if any.
 bool tune_lookup(const qs_plan *p, int &tile, int &splits);
 int grouped_tile(int64_t M, int64_t N, int ct);
+// Grouped-launch shape classes and the autotune cache's raw entries.
+std::string group_shape_key(const qs_plan *p);
+bool tune_find(const std::string &key, int &tile, int &splits);
+void tune_store(const std::string &key, int tile, int splits, double ms);
+void tune_erase(const std::string &key);
 // Split precision (qs_split.hip).
 static constexpr int kSplitSamples = 256;
 int split_moduli_for(double target, int ct, int64_t K);
 const std::vector<int> &split_moduli_table();
 qs_status split_prepare(qs_plan *p);
 hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t s);
+// Guarded split execute (split_report_on == 2): report, escalate, fall back.
+qs_status run_split_guarded(qs_plan *p, const GemmArgs &g, hipStream_t s);
 GemmArgs make_args(const qs_plan *p, const void *A, int64_t a_base, const void *B, int64_t b_base, void *C,
                    int64_t c_base);
 }  // namespace qs

@@ -339,6 +339,7 @@ struct CrtGemmArgs {
   int64_t MB, NB, KB;        // Mp/16, Np/16, Kp/64
   int64_t tiles_m, tiles_n;
   int n;
+  int xcds;
   int m[kMaxModuli];
   float invm[kMaxModuli];
 };
@@ -365,7 +366,7 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void crt_gemm_kernel(const CrtGemm
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = wave % WM, wn = wave / WM;
   const int64_t tiles = a.tiles_m * a.tiles_n;
-  const int64_t bid = xcd_swizzle(blockIdx.x, (int64_t)gridDim.x);
+  const int64_t bid = xcd_swizzle(blockIdx.x, (int64_t)gridDim.x, a.xcds);
   const int l = (int)(bid / tiles);
   const int64_t tt = bid % tiles;
   const int64_t grp = tt / (4 * a.tiles_n), in = tt % (4 * a.tiles_n);
@@ -772,13 +773,14 @@ qs_status split_prepare(qs_plan *p) {
   return QS_OK;
 }
 
-hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t s) {
+// Error report, part 1 (before C is written): the sample positions and the
+// old C at each of them, in the workspace.
+static hipError_t split_sample(qs_plan *p, const GemmArgs &g, hipStream_t s, int &nsmp) {
   const SplitLayout L = split_layout(p);
   char *base = (char *)p->d_split;
   SplitSample *smp = (SplitSample *)(base + L.smp);
-  double *rep = (double *)(base + L.rep);
-  int nsmp = 0;
-  if (report) {
+  nsmp = 0;
+  {
     nsmp = (int)std::min<int64_t>(kSplitSamples, p->M * p->N);
     // Deterministic sample positions: a fixed stride walk over the M x N grid.
     std::vector<SplitSample> h(nsmp);
@@ -795,7 +797,13 @@ hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t 
     e = hipStreamSynchronize(s);   // `h` goes out of scope
     if (e != hipSuccess) return e;
   }
+  return hipSuccess;
+}
 
+// The split-precision contraction itself, with the plan's moduli.
+static hipError_t split_compute(qs_plan *p, const GemmArgs &g, hipStream_t s) {
+  const SplitLayout L = split_layout(p);
+  char *base = (char *)p->d_split;
   const int n = p->moduli;
   const CrtConst c = crt_constants(n);
   int eT;
@@ -834,6 +842,7 @@ hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t 
   ga.NB = p->split_Np / 16;
   ga.KB = p->split_Kp / 64;
   ga.n = n;
+  ga.xcds = device_facts().xcds;
   for (int l = 0; l < kMaxModuli; ++l) {
     ga.m[l] = l < n ? split_moduli_table()[l] : 1;
     ga.invm[l] = 1.0f / (float)ga.m[l];
@@ -856,11 +865,23 @@ hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t 
   const dim3 rgrid((unsigned)(ga.MB * ga.NB / 16));
   if (p->c_mfast) hipLaunchKernelGGL(crt_rec_kernel<true>, rgrid, dim3(256), 0, s, ra, c);
   else hipLaunchKernelGGL(crt_rec_kernel<false>, rgrid, dim3(256), 0, s, ra, c);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || !report) return e;
+  return hipGetLastError();
+}
 
+static double split_target(const qs_plan *p) {
+  const double t = p->desc.target_rel_err;
+  return t > 0 ? t : (p->desc.compute_type == QS_F64 ? 1e-13 : 1e-6);
+}
+
+// Error report, part 2 (after C is written): Dot2 at the samples, reduced on
+// the host into the plan's report.
+static hipError_t split_report(qs_plan *p, const GemmArgs &g, hipStream_t s, int nsmp) {
+  const SplitLayout L = split_layout(p);
+  char *base = (char *)p->d_split;
+  SplitSample *smp = (SplitSample *)(base + L.smp);
+  double *rep = (double *)(base + L.rep);
   hipLaunchKernelGGL(split_report_kernel, dim3(nsmp), dim3(64), 0, s, g, smp, rep);
-  e = hipGetLastError();
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   std::vector<double> out(3 * nsmp);
   e = hipMemcpyAsync(out.data(), rep, sizeof(double) * 3 * nsmp, hipMemcpyDeviceToHost, s);
@@ -877,7 +898,100 @@ hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t 
   p->split_report.samples = nsmp;
   p->split_report.rel_fro = den > 0 ? std::sqrt(num / den) : std::sqrt(num);
   p->split_report.max_rel = mx;
+  p->split_report.exceeded = p->split_report.rel_fro > split_target(p) ? 1 : 0;
   return hipSuccess;
+}
+
+hipError_t launch_split(qs_plan *p, const GemmArgs &g, bool report, hipStream_t s) {
+  int nsmp = 0;
+  hipError_t e = hipSuccess;
+  if (report) {
+    p->split_report.escalations = 0;
+    p->split_report.fallback = 0;
+    e = split_sample(p, g, s, nsmp);
+  }
+  if (e == hipSuccess) e = split_compute(p, g, s);
+  if (e == hipSuccess && report) e = split_report(p, g, s, nsmp);
+  return e;
+}
+
+// The workspace for `n` moduli (its layout depends on n).
+static bool split_resize(qs_plan *p, int n, hipStream_t s) {
+  (void)hipStreamSynchronize(s);
+  (void)hipStreamSynchronize(p->split_side);
+  ws_free(p, p->d_split, (size_t)p->split_bytes);
+  p->d_split = nullptr;
+  p->split_bytes = 0;
+  p->moduli = n;
+  const SplitLayout L = split_layout(p);
+  p->d_split = ws_alloc(p, L.total);
+  if (!p->d_split) return false;
+  p->split_bytes = (int64_t)L.total;
+  return true;
+}
+
+// Guarded execute: the sampled error relative to C decides.  Above the
+// target, the execute is redone from C as it was with more moduli (each adds
+// about 3.9 bits to T, which the error scales with), up to kMaxModuli, and
+// then with the plain MFMA kernel in the compute type.
+qs_status run_split_guarded(qs_plan *p, const GemmArgs &g, hipStream_t s) {
+  auto dev_err = [](hipError_t e) { return fail(QS_ERR_DEVICE, std::string("execute!: split: ") + hipGetErrorString(e)); };
+  const double target = split_target(p);
+  p->split_report.escalations = 0;
+  p->split_report.fallback = 0;
+  const int eb = elem_bytes(g.c_type);
+  const int64_t clo = g.c_base + p->mC_lo + p->nC_lo, chi = g.c_base + p->mC_hi + p->nC_hi;
+  const size_t cbytes = (size_t)(chi - clo + 1) * eb;
+  char *cspan = (char *)g.C + clo * eb;
+  const bool keep = g.beta_case != 0;
+  if (keep && (int64_t)cbytes > p->cbackup_bytes) {
+    ws_free(p, p->d_cbackup, (size_t)p->cbackup_bytes);
+    p->cbackup_bytes = 0;
+    p->d_cbackup = ws_alloc(p, cbytes);
+    if (!p->d_cbackup) return fail(QS_ERR_MEMORY, "execute!: split: buffer for C's span");
+    p->cbackup_bytes = (int64_t)cbytes;
+  }
+  hipError_t e = hipSuccess;
+  if (keep) e = hipMemcpyAsync(p->d_cbackup, cspan, cbytes, hipMemcpyDeviceToDevice, s);
+  auto restore = [&]() {
+    return keep ? hipMemcpyAsync(cspan, p->d_cbackup, cbytes, hipMemcpyDeviceToDevice, s) : hipSuccess;
+  };
+  int nsmp = 0;
+  if (e == hipSuccess) e = split_sample(p, g, s, nsmp);
+  if (e == hipSuccess) e = split_compute(p, g, s);
+  if (e == hipSuccess) e = split_report(p, g, s, nsmp);
+  if (e != hipSuccess) return dev_err(e);
+  while (p->split_report.rel_fro > target && p->moduli < kMaxModuli) {
+    const double f = p->split_report.rel_fro / target;
+    const int extra = std::max(1, (int)std::ceil(std::log2(2.0 * f) / 3.9));
+    if (!split_resize(p, std::min(kMaxModuli, p->moduli + extra), s)) break;
+    ++p->split_report.escalations;
+    e = restore();
+    if (e == hipSuccess) e = split_sample(p, g, s, nsmp);
+    if (e == hipSuccess) e = split_compute(p, g, s);
+    if (e == hipSuccess) e = split_report(p, g, s, nsmp);
+    if (e != hipSuccess) return dev_err(e);
+  }
+  if (!p->d_split) {   // a resize failed: the workspace is gone; the plain kernel needs none
+    if (!split_resize(p, p->moduli, s)) return fail(QS_ERR_MEMORY, "execute!: split workspace");
+    e = restore();
+    if (e == hipSuccess) e = split_sample(p, g, s, nsmp);
+    if (e != hipSuccess) return dev_err(e);
+    p->split_report.rel_fro = 2 * target + 1;
+  }
+  if (p->split_report.rel_fro > target) {
+    GemmArgs gp = g;
+    gp.tile = choose_tile(p->M, p->N, p->desc.compute_type);
+    gp.k_splits = 1;
+    gp.k_chunk = p->K;
+    gp.partials = nullptr;
+    e = restore();
+    if (e == hipSuccess) e = launch_gemm(gp, s);
+    if (e == hipSuccess) e = split_report(p, g, s, nsmp);
+    if (e != hipSuccess) return dev_err(e);
+    p->split_report.fallback = 1;
+  }
+  return QS_OK;
 }
 
 }  // namespace qs

@@ -41,6 +41,38 @@ std::string shape_key(const qs_plan *p) {
   return os.str();
 }
 
+// The shape class of a member of a grouped launch: types, gather
+// directions and octave buckets of M, N, K (coarser than a one-contraction
+// plan's: a group holds many shapes, and each class is timed on its own).
+static int octave(int64_t x) { return x <= 1 ? 0 : (int)std::floor(std::log2((double)x)); }
+
+std::string group_shape_key(const qs_plan *p) {
+  std::ostringstream os;
+  const qs_contract_desc &d = p->desc;
+  os << "g:" << d.compute_type << ',' << d.a_type << ',' << d.b_type << ',' << d.c_type << ',' << p->a_kfast << ','
+     << p->b_kfast << ',' << octave(p->M) << ',' << octave(p->N) << ',' << octave(p->K);
+  return os.str();
+}
+
+bool tune_find(const std::string &key, int &tile, int &splits) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(key);
+  if (it == g_tune.end()) return false;
+  tile = it->second.tile;
+  splits = it->second.splits;
+  return true;
+}
+
+void tune_store(const std::string &key, int tile, int splits, double ms) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[key] = TuneEntry{tile, splits, ms};
+}
+
+void tune_erase(const std::string &key) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.erase(key);
+}
+
 bool tune_lookup(const qs_plan *p, int &tile, int &splits) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find(shape_key(p));
@@ -155,7 +187,8 @@ qs_status qs_autotune_save(const char *path) {
   std::ofstream f(path);
   if (!f) return fail(QS_ERR_ARGUMENT, std::string("autotune_save: cannot open ") + path);
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  f << "# qs_amd autotune cache: compute,a,b,c,a_kfast,b_kfast,mbucket,nbucket,kbucket tile splits ms\n";
+  f << "# qs_amd autotune cache: compute,a,b,c,a_kfast,b_kfast,mbucket,nbucket,kbucket tile splits ms"
+       " (g: grouped-launch classes, octave buckets)\n";
   for (const auto &kv : g_tune) f << kv.first << ' ' << kv.second.tile << ' ' << kv.second.splits << ' ' << kv.second.ms << '\n';
   return QS_OK;
 }

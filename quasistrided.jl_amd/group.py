@@ -76,6 +76,16 @@ class GroupPlan:
         s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
         _check(_lib.lib().qs_group_execute(self._h, _coef(alphas, n), _coef(betas, n), C.c_void_p(s)))
 
+    def autotune(self, reps=3, stream=None) -> dict:
+        """Time every grouped tile class per shape class of the members (as
+        grouped launches writing a scratch C), keep the fastest in the autotune
+        cache and rebuild the group with it."""
+        import torch
+        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        r = _lib.QSGroupTuneResult()
+        _check(_lib.lib().qs_group_autotune(self._h, C.c_void_p(s), int(reps), C.byref(r)))
+        return {f: getattr(r, f) for f, _ in _lib.QSGroupTuneResult._fields_ if not f.startswith("_")}
+
     def capture(self, repeats=1, alphas=1.0, betas=0.0, stream=None) -> Graph:
         import torch
         n = len(self.plans)

@@ -190,12 +190,16 @@ def autotune(plan: ContractPlan, reps: int = 5, stream=None) -> dict:
     return {f: getattr(r, f) for f, _ in _lib.QSTuneResult._fields_ if not f.startswith("_")}
 
 
-def set_error_report(plan: ContractPlan, on: bool = True):
+def set_error_report(plan: ContractPlan, on=True):
     """Split-precision plans: sample-check every execute against a Dot2
-    evaluation of the same entries (costs a stream synchronization)."""
-    st = _lib.lib().qs_plan_set_error_report(plan.handle, 1 if on else 0)
+    evaluation of the same entries (costs a stream synchronization).
+    `on="guard"`: an execute whose sampled error exceeds the target is redone
+    with more moduli, then with the plain MFMA kernel."""
+    mode = 2 if on == "guard" else (1 if on else 0)
+    st = _lib.lib().qs_plan_set_error_report(plan.handle, mode)
     if st != 0:
         raise_status(st, _lib.last_error())
+    _refresh_info(plan)
 
 
 def error_report(plan: ContractPlan) -> dict:
@@ -203,7 +207,7 @@ def error_report(plan: ContractPlan) -> dict:
     st = _lib.lib().qs_plan_get_error_report(plan.handle, C.byref(r))
     if st != 0:
         raise_status(st, _lib.last_error())
-    return {f: getattr(r, f) for f, _ in _lib.QSErrorReport._fields_}
+    return {f: getattr(r, f) for f, _ in _lib.QSErrorReport._fields_ if not f.startswith("_")}
 
 
 def autotune_save(path: str):

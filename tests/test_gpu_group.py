@@ -114,3 +114,48 @@ def test_group_mixed_types_run_standalone(oracle):
     for i, p in enumerate(plans):
         want = oracle_of_plan(oracle, p, 1.5, 0.5, storage_np(p.Astorage), storage_np(p.Bstorage), starts[i])
         assert rel_fro(storage_np(views[i].parent), want) <= 1e-13
+
+
+def test_group_autotune_keeps_results_and_c_and_persists(tmp_path):
+    case("group autotune: members' C untouched, results unchanged, classes saved and reloaded")
+    rng = np.random.default_rng(17)
+    shapes = dims(rng, 20, lo=16, hi=300)
+    views, plans = [], []
+    for (a, ap, b, s) in shapes:
+        Cv, L, P = mps_env(rng, np.float64, a, ap, b, s)
+        views.append(Cv)
+        plans.append(qs.plan_contract(Cv, L, (1, 2, 3), P, (3, 4, 5), (1, 2, 4, 5)))
+    qs.autotune_clear()
+    grp = qs.GroupPlan(plans)
+    grp.execute(1.0, 0.0)
+    torch.cuda.synchronize()
+    first = [storage_np(v.parent) for v in views]
+    for v in views:
+        v.parent.fill_(3.0)
+    res = grp.autotune(reps=2)
+    torch.cuda.synchronize()
+    print("[group tune]", res, flush=True)
+    assert res["classes"] >= 1 and res["candidates"] == 3 and res["after_ms"] <= res["before_ms"]
+    assert all(torch.all(v.parent == 3.0).item() for v in views)       # tuning writes a scratch C
+    grp.execute(1.0, 0.0)
+    torch.cuda.synchronize()
+    for i, v in enumerate(views):
+        assert np.array_equal(storage_np(v.parent), first[i]), i        # same bits with the tuned tiles
+    path = str(tmp_path / "tune.txt")
+    qs.autotune_save(path)
+    text = open(path).read()
+    n_g = sum(line.startswith("g:") for line in text.splitlines())
+    # the tuned classes are kept only when the whole group ran faster with them
+    assert n_g == (res["classes"] if res["after_ms"] < res["before_ms"] else 0), text
+    qs.autotune_clear()
+    qs.autotune_load(path)
+    again = qs.GroupPlan(plans)          # built from the reloaded classes
+    for v in views:
+        v.parent.fill_(float("nan"))
+    again.execute(1.0, 0.0)
+    torch.cuda.synchronize()
+    for i, v in enumerate(views):
+        assert np.array_equal(storage_np(v.parent), first[i]), i
+    again.release()
+    grp.release()
+    qs.autotune_clear()

@@ -186,3 +186,81 @@ def test_split_is_bitwise_reproducible():
     qs.execute(plan, 1.0, 0.0)
     torch.cuda.synchronize()
     assert np.array_equal(storage_np(Cv.parent), first)
+
+
+def _cancelling(rng, M, K, N, delta):
+    """A = [X, X], B = [Y; -Y + delta E]: C = delta X E, with |C_ij| about
+    (90 / delta)^-1 of ||A_i|| ||B_j|| at K = 4096 (cancellation)."""
+    h = K // 2
+    X = rng.standard_normal((M, h))
+    Y = rng.standard_normal((h, N))
+    E = rng.standard_normal((h, N))
+    A = np.concatenate([X, X], axis=1)
+    B = np.concatenate([Y, -Y + delta * E], axis=0)
+    exact = delta * (X @ E)    # no cancellation in X E: float64 error ~1e-16 of it
+    Av = SV(torch.from_numpy(A.reshape(-1, order="F").copy()).to(DEV), (M, K), (1, M))
+    Bv = SV(torch.from_numpy(B.reshape(-1, order="F").copy()).to(DEV), (K, N), (1, K))
+    return A, B, Av, Bv, exact
+
+
+def test_split_under_cancellation_normwise_bound_and_guard():
+    case("split F64 under cancellation (|C| << |A||B|): normwise bound, report flags it, guarded mode meets 1e-13")
+    rng = np.random.default_rng(15)
+    M, K, N = 192, 4096, 160
+    A, B, Av, Bv, exact = _cancelling(rng, M, K, N, 0.02)
+    norms = np.linalg.norm(A, axis=1)[:, None] * np.linalg.norm(B, axis=0)[None, :]
+    print(f"[cancel] median ||A_i|| ||B_j|| / |C_ij| = {np.median(norms / np.abs(exact)):.0f}", flush=True)
+
+    def run(mode, beta=0.0, c0=None):
+        c0 = np.zeros(M * N) if c0 is None else c0
+        Cv = SV(torch.from_numpy(c0.copy()).to(DEV), (M, N), (1, M))
+        plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split" if mode is not None else "plain")
+        if mode is not None:
+            qs.set_error_report(plan, mode)
+        qs.execute(plan, 1.0, beta)
+        torch.cuda.synchronize()
+        got = storage_np(Cv.parent).reshape(N, M).T
+        rep = qs.error_report(plan) if mode is not None else None
+        return got, rep, plan
+
+    plain, _, _ = run(None)
+    plain_err = rel_fro(plain, exact)
+    got, rep, plan = run(True)
+    err = rel_fro(got, exact)
+    print(f"[cancel] plain FP64 rel err {plain_err:.3e}; split moduli={plan.info['moduli']} rel err {err:.3e} "
+          f"report {rep}", flush=True)
+    # the normwise bound holds entry by entry
+    assert np.all(np.abs(got - exact) <= 1e-13 * norms)
+    # relative to C the target is missed, and the report says so (its sampled error tracks the true one)
+    assert err > 1e-13 and rep["exceeded"] == 1, (err, rep)
+    assert 0.3 * err <= rep["rel_fro"] <= 3.0 * err, (err, rep)
+    # guarded: redone with more moduli, from C as it was (beta != 0 here)
+    c0 = rng.standard_normal(M * N)
+    want = exact + 0.5 * c0.reshape(N, M).T
+    got, rep, plan = run("guard", beta=0.5, c0=c0)
+    gerr = rel_fro(got, want)
+    print(f"[cancel] guarded moduli={plan.info['moduli']} err {gerr:.3e} report {rep}", flush=True)
+    assert gerr <= 1e-13 and rep["exceeded"] == 0 and rep["escalations"] >= 1 and rep["fallback"] == 0, (gerr, rep)
+    assert rep["moduli"] > 13                                # the raised count stays with the plan
+
+
+def test_split_guard_falls_back_to_plain_past_15_moduli():
+    case("split F64 guarded, cancellation beyond what 15 moduli resolve: the plain kernel finishes the execute")
+    rng = np.random.default_rng(16)
+    M, K, N = 96, 4096, 80
+    A, B, Av, Bv, exact = _cancelling(rng, M, K, N, 2e-3)
+    Cv = SV(torch.from_numpy(np.zeros(M * N)).to(DEV), (M, N), (1, M))
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split", target_error=1e-15)
+    qs.set_error_report(plan, "guard")
+    qs.execute(plan, 1.0, 0.0)
+    torch.cuda.synchronize()
+    rep = qs.error_report(plan)
+    got = storage_np(Cv.parent).reshape(N, M).T
+    Cp = SV(torch.from_numpy(np.zeros(M * N)).to(DEV), (M, N), (1, M))
+    pp = qs.plan_contract(Cp, Av, (1, 2), Bv, (2, 3), (1, 3), kernel=qs.MFMAKernel(k_splits=1))
+    assert pp.info["k_splits"] == 1
+    qs.execute(pp, 1.0, 0.0)
+    torch.cuda.synchronize()
+    print(f"[cancel] fallback report {rep} moduli {plan.info['moduli']}", flush=True)
+    assert rep["fallback"] == 1 and plan.info["moduli"] == 15, rep
+    assert np.array_equal(got, storage_np(Cp.parent).reshape(N, M).T)   # the plain kernel's bits
