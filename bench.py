@@ -379,9 +379,18 @@ def bench_grouped(qs, torch, dev, cplx, reps=5):
     e1.record()
     torch.cuda.synchronize()
     t_graph = e0.elapsed_time(e1) * 1e-3 / reps
+    # The batch's roofline: each member at min(FP64 MFMA peak, its bytes at
+    # 8 TB/s) -- sum over members of max(flops / peak, bytes / bandwidth),
+    # bytes = A + B + C (write only, beta = 0).
+    esz = 16 if cplx else 8
+    roof = sum(max((8 if cplx else 2) * p.info["m"] * p.info["n"] * p.info["k"] / PEAK_FP64,
+                   (p.info["m"] * p.info["k"] + p.info["k"] * p.info["n"] + p.info["m"] * p.info["n"]) * esz / PEAK_HBM)
+               for p in plans)
     out = {
         "contractions": n,
         "gflop_per_sweep": round(flops / 1e9, 2),
+        "roofline_ms": round(roof * 1e3, 4),
+        "graph_roofline_frac": round(roof / t_graph, 4),
         "per_launch_contractions_per_s": round(n / t_plain, 1),
         "grouped_contractions_per_s": round(n / t_group, 1),
         "graph_contractions_per_s": round(n / t_graph, 1),
@@ -686,6 +695,8 @@ def main():
         with open(traffic_src) as f:
             traffic_bytes = json.load(f).get("traffic_bytes_per_launch")
 
+    # Measured peaks beside the datasheet ones (FP64 MFMA probe, device copy).
+    peaks = qs.measure_peaks()
     result = {
         "metric": "GFLOP/s (plain FP64 block-scatter contraction)",
         "value": round(value, 1),
@@ -712,12 +723,17 @@ def main():
             "peak": 78.6,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64, 4),
+            "frac_wall": round(flops / (wall / args.steps) / PEAK_FP64, 4),
+            "peak_measured": round(peaks["fp64_mfma_tflops"], 2),
+            "frac_of_measured_peak": round(achieved / 1e12 / peaks["fp64_mfma_tflops"], 4),
+            "copy_gbs_measured": round(peaks["copy_gbs"], 1),
             "traffic": traffic_bytes,
             "launch_period_ms": round(period * 1e3, 5),
             "kernel_ms_mean": round(kern_mean * 1e3, 5),
             "kernel_ms_median": round(kern_med * 1e3, 5),
             "note": "achieved = flops per launch / launch_period_ms (HIP events at both ends of the timed "
-                    "region, gaps included); kernel_ms_* from events around each launch in a second pass",
+                    "region, gaps included); frac_wall from ms_per_step (wall clock); kernel_ms_* from events "
+                    "around each launch in a second pass; peak_measured: FP64 MFMA probe (qs_target_measure)",
         },
     }
 

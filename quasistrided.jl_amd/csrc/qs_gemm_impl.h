@@ -519,11 +519,14 @@ struct TileGemm {
   // allow (one storage type; real tiles up to 64 x 128, complex up to
   // 32 x 64); the grouped kernel, whose tile classes share one register
   // allocation, passes false.
-  template <bool DEEPEN = true>
+  // DEEPEN 2: two stages for any tile (a caller whose register budget is
+  // known to hold them).
+  template <int DEEPEN = 1>
   __device__ static void run(const GemmArgs &g, int64_t tm, int64_t tn, int split, R *smem) {
     // Complex tiles up to 32 x 64 too: their gathers are small (ComplexF64
     // 32 x 64 x 16: 166 VGPRs, 139 -> 130 us at 1024^3).
-    constexpr bool DEEP = DEEPEN && !MIXED && (CPLX ? BM * BN <= 32 * 64 : BM * BN <= 64 * 128);
+    constexpr bool DEEP =
+        !MIXED && (DEEPEN == 2 || (DEEPEN == 1 && (CPLX ? BM * BN <= 32 * 64 : BM * BN <= 64 * 128)));
     typedef typename Acc4<R>::type V;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;

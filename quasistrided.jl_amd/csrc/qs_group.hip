@@ -49,12 +49,19 @@ struct GroupSmem {
 // coef: per item {alpha_re, alpha_im, beta_re, beta_im}, indexed by item.index.
 // cycles (the autotuner's measurement, null otherwise): per member, the
 // workgroup time its tiles took, in ticks of the constant 100 MHz clock.
+//
+// Round r of the persistent loop covers tiles [r G, (r + 1) G) of the G
+// workgroups; within a round each XCD takes a contiguous run of them
+// (xcd_swizzle), so the tiles one XCD runs at once are neighbours of one
+// contraction, which share its A rows or B columns in that XCD's L2.
 template <typename R, bool CPLX, bool AKF, bool BKF>
 __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__restrict__ items, int n_items,
                                                           int64_t total_tiles, const double *__restrict__ coef,
-                                                          unsigned long long *cycles) {
+                                                          unsigned long long *cycles, int xcds) {
   __shared__ R smem[GroupSmem<R, CPLX, AKF, BKF>::value];
-  for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+  const int64_t G = gridDim.x, slot = xcd_swizzle(blockIdx.x, G, xcds);
+  for (int64_t r0 = 0; r0 + slot < total_tiles; r0 += G) {
+    const int64_t t = r0 + slot;
     uint64_t t0 = 0;
     if (cycles) {
       __syncthreads();
@@ -76,10 +83,14 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
     g.beta_im = c[3];
     g.beta_case = (c[2] == 0 && c[3] == 0) ? 0 : (c[2] == 1 && c[3] == 0) ? 1 : 2;
     if (c[0] == 0 && c[1] == 0) g.K = 0;   // alpha == 0: C = beta C, A and B never read
+    // One stage in flight: the three tile bodies share one register
+    // allocation.  (Complex types with 64 x 64 and 32 x 64 tiles only, each
+    // with two stages in flight, measured slower on the 512-member ComplexF64
+    // sweep: 131-132k against 147k contractions per second.)
     switch (items[lo].cls) {
-      case TILE_128x64: TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::template run<false>(g, tm, tn, 0, smem); break;
-      case TILE_32x64: TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::template run<false>(g, tm, tn, 0, smem); break;
-      default: TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::template run<false>(g, tm, tn, 0, smem); break;
+      case TILE_128x64: TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
+      case TILE_32x64: TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
+      default: TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
     }
     if (cycles) {
       __syncthreads();
@@ -91,7 +102,8 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
 template <typename R, bool CPLX, bool AKF, bool BKF>
 static hipError_t launch_grouped_t(const GroupItem *items, int n, int64_t total, const double *coef, int grid,
                                    hipStream_t s, unsigned long long *cycles) {
-  hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF>), dim3(grid), dim3(256), 0, s, items, n, total, coef, cycles);
+  hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF>), dim3(grid), dim3(256), 0, s, items, n, total, coef, cycles,
+                     device_facts().xcds);
   return hipGetLastError();
 }
 
@@ -114,6 +126,12 @@ static hipError_t launch_grouped(int ct, int akf, int bkf, const GroupItem *item
     case QS_C64: return launch_grouped_rc<double, true>(akf, bkf, items, n, total, coef, grid, s, cy);
     default: return launch_grouped_rc<float, true>(akf, bkf, items, n, total, coef, grid, s, cy);
   }
+}
+
+// The grouped kernel's tile classes: 128 x 64, 64 x 64 and 32 x 64.
+static int grouped_class(int cls, int ct) {
+  (void)ct;
+  return (cls == TILE_128x64 || cls == TILE_32x64) ? cls : TILE_64x64;
 }
 
 // Workgroup tile of one contraction inside a group: the grid is already
@@ -213,7 +231,7 @@ static int member_class(const qs_group_plan *g, size_t i) {
   int cls = g->tile_override[i], splits = 1;
   if (cls < 0 && !tune_find(group_shape_key(p), cls, splits)) cls = -1;
   if (cls < 0) cls = grouped_tile(p->M, p->N, p->desc.compute_type);
-  return (cls == TILE_128x64 || cls == TILE_32x64) ? cls : TILE_64x64;
+  return grouped_class(cls, p->desc.compute_type);
 }
 
 // Device tables of grouped launches for the members with cls[i] >= 0 (one
@@ -233,7 +251,7 @@ static qs_status make_launches(qs_group_plan *g, const std::vector<int> &cls, vo
     GroupItem it;
     std::memset(&it, 0, sizeof it);
     it.g = a;
-    it.cls = cls[i];
+    it.cls = grouped_class(cls[i], p->desc.compute_type);
     const TileShape ts = tile_shape(it.cls, p->desc.compute_type, p->a_kfast, p->b_kfast);
     it.tiles_m = (p->M + ts.bm - 1) / ts.bm;
     it.tile_start = it.tiles_m * ((p->N + ts.bn - 1) / ts.bn);   // tile count, prefix-summed below
@@ -422,7 +440,7 @@ qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vect
     GroupItem it;
     std::memset(&it, 0, sizeof it);
     it.g = args[i];
-    it.cls = cls[i];
+    it.cls = grouped_class(cls[i], it.g.compute_type);
     const TileShape ts = tile_shape(it.cls, it.g.compute_type, it.g.a_kfast, it.g.b_kfast);
     it.tiles_m = (it.g.M + ts.bm - 1) / ts.bm;
     it.tile_start = it.tiles_m * ((it.g.N + ts.bn - 1) / ts.bn);
@@ -601,8 +619,7 @@ qs_status qs_group_autotune(qs_group_plan *g, void *stream, int32_t reps, qs_gro
       fixed[i] = before_cls[i];
       continue;
     }
-    const int h = grouped_tile(p->M, p->N, p->desc.compute_type);
-    heur[i] = (h == TILE_128x64 || h == TILE_32x64) ? h : TILE_64x64;
+    heur[i] = grouped_class(grouped_tile(p->M, p->N, p->desc.compute_type), p->desc.compute_type);
     classes[group_shape_key(p)].push_back(i);
   }
   if (classes.empty()) return QS_OK;
