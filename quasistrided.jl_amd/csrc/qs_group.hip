@@ -136,10 +136,14 @@ static int grouped_class(int cls, int ct) {
 
 // Workgroup tile of one contraction inside a group: the grid is already
 // full of other contractions' tiles, so take the tile that wastes the least
-// padded work, weighted by how much slower the smaller register tiles run.
+// padded work, weighted by how much slower the smaller register tiles run
+// per padded element.  The weights are measured: on the 512-contraction MPS
+// sweep (FP64 and complex FP64) they give the whole group the time of
+// all-128x64 tiles, which a 64x64 weight of 1.2 and a 32x64 weight of 1.7
+// did not (1-8% slower), and they keep the smaller tiles for small members.
 int grouped_tile(int64_t M, int64_t N, int ct) {
   struct Cand { int id, bm, bn; double cost; };
-  const Cand cands[] = {{TILE_128x64, 128, 64, 1.0}, {TILE_64x64, 64, 64, 1.2}, {TILE_32x64, 32, 64, 1.7}};
+  const Cand cands[] = {{TILE_128x64, 128, 64, 1.0}, {TILE_64x64, 64, 64, 1.35}, {TILE_32x64, 32, 64, 2.0}};
   int best = TILE_64x64;
   double best_cost = 1e300;
   for (const Cand &c : cands) {
