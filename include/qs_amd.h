@@ -456,6 +456,20 @@ typedef struct qs_alloc_stats {
 
 qs_status qs_alloc_stats_get(qs_alloc_stats *out);
 
+/* ---- fault injection --------------------------------------------------------
+ * No counterpart in the reference, which has no allocation that can fail
+ * partway (its errors are argument errors raised before any write).  For
+ * tests of the library's own error paths: arms a one-shot failure of the
+ * (after + 1)-th device allocation (QS_FAULT_DEVICE_ALLOC) or pinned host
+ * allocation (QS_FAULT_PINNED_ALLOC) the library makes from now on.  That
+ * allocation returns out-of-memory without calling the runtime, so the call
+ * that made it returns QS_ERR_MEMORY (or QS_ERR_DEVICE where the allocation
+ * is not the caller's memory) with nothing written to C and nothing leaked.
+ * after < 0 disarms.  One counter per kind, shared by every thread. */
+typedef enum qs_fault_kind { QS_FAULT_DEVICE_ALLOC = 0, QS_FAULT_PINNED_ALLOC = 1 } qs_fault_kind;
+
+qs_status qs_fault_inject(int32_t kind, int64_t after);
+
 /* Host-side offset enumeration, exactly `fill_offsets!`
  * (src/layout/axis_group.jl:87-145) for a two-map group: writes `count`
  * offsets of coordinates [first, first+count) into out0/out1.  Exposed so a

@@ -20,6 +20,7 @@ QS_ERR_DEVICE, QS_ERR_MEMORY, QS_ERR_UNSUPPORTED = 5, 6, 7
 QS_F32, QS_F64, QS_C32, QS_C64 = 0, 1, 2, 3
 QS_MEM_DEVICE, QS_MEM_HOST = 0, 1
 QS_PLAIN, QS_SPLIT_INT8 = 0, 1
+QS_FAULT_DEVICE_ALLOC, QS_FAULT_PINNED_ALLOC = 0, 1
 
 PATH_NAMES = {0: "empty", 1: "scale", 2: "gemm", 3: "dot", 4: "outer", 5: "skinny", 6: "split"}
 
@@ -169,6 +170,7 @@ SIGNATURES = {
     "qs_fill_offsets": (C.c_int, [C.POINTER(QSGroup), C.c_int64, C.c_int64, C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64)]),
     "qs_alloc_stats_get": (C.c_int, [C.POINTER(QSAllocStats)]),
+    "qs_fault_inject": (C.c_int, [C.c_int32, C.c_int64]),
     "qs_contract_noalloc": (C.c_int, [C.POINTER(QSContractDesc), C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 

@@ -40,7 +40,23 @@ qs_status fail_fmt(qs_status code, const char *fmt, ...) {
 static std::atomic<int64_t> g_dev_allocs{0}, g_dev_frees{0}, g_dev_bytes{0}, g_pin_allocs{0}, g_pin_frees{0};
 static std::atomic<int64_t> g_bridge_allocs{0}, g_bridge_frees{0};
 
+// Armed one-shot failures (qs_fault_inject): the number of allocations of the
+// kind still to succeed before one fails; negative when disarmed.
+static std::atomic<int64_t> g_fault_left[2] = {{-1}, {-1}};
+
+static bool fault_fires(int kind) {
+  std::atomic<int64_t> &c = g_fault_left[kind];
+  int64_t v = c.load();
+  while (v >= 0)
+    if (c.compare_exchange_weak(v, v - 1)) return v == 0;
+  return false;
+}
+
 hipError_t dev_malloc(void **p, size_t bytes) {
+  if (fault_fires(QS_FAULT_DEVICE_ALLOC)) {
+    *p = nullptr;
+    return hipErrorOutOfMemory;
+  }
   const hipError_t e = hipMalloc(p, bytes);
   if (e == hipSuccess) {
     g_dev_allocs.fetch_add(1, std::memory_order_relaxed);
@@ -53,6 +69,10 @@ hipError_t dev_free(void *p) {
   return hipFree(p);
 }
 hipError_t pinned_malloc(void **p, size_t bytes, unsigned int flags) {
+  if (fault_fires(QS_FAULT_PINNED_ALLOC)) {
+    *p = nullptr;
+    return hipErrorOutOfMemory;
+  }
   const hipError_t e = hipHostMalloc(p, bytes, flags);
   if (e == hipSuccess) g_pin_allocs.fetch_add(1, std::memory_order_relaxed);
   return e;
@@ -864,6 +884,13 @@ qs_status qs_alloc_stats_get(qs_alloc_stats *out) {
   out->pinned_frees = g_pin_frees.load();
   out->bridge_allocs = g_bridge_allocs.load();
   out->bridge_frees = g_bridge_frees.load();
+  return QS_OK;
+}
+
+qs_status qs_fault_inject(int32_t kind, int64_t after) {
+  if (kind != QS_FAULT_DEVICE_ALLOC && kind != QS_FAULT_PINNED_ALLOC)
+    return fail_fmt(QS_ERR_ARGUMENT, "qs_fault_inject: unknown kind %d", (int)kind);
+  g_fault_left[kind].store(after < 0 ? -1 : after);
   return QS_OK;
 }
 

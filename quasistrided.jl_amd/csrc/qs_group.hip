@@ -193,6 +193,11 @@ struct qs_group_plan {
     }
   };
   std::vector<Storages> held;
+  // The launch tables match `held` and every member is in them.  Cleared
+  // while they are rebuilt, so a rebuild that failed (an allocation) is
+  // retried, or its error returned, by the next execute, capture or autotune
+  // instead of running with some members missing.
+  bool tables_ok = false;
 };
 
 struct qs_graph {
@@ -299,6 +304,7 @@ static void free_launch_list(std::vector<Launch> &ls) {
 
 // (Re)build the device tables from the plans and their tile classes.
 static qs_status build_tables(qs_group_plan *g) {
+  g->tables_ok = false;
   free_launches(g);
   g->held.clear();
   for (const qs_plan *p : g->plans) g->held.push_back(storages_of(p));
@@ -314,6 +320,7 @@ static qs_status build_tables(qs_group_plan *g) {
   }
   qs_status st = make_launches(g, cls, nullptr, g->launches);
   if (st != QS_OK) free_launches(g);
+  else g->tables_ok = true;
   return st;
 }
 
@@ -395,7 +402,7 @@ static void build_batch_args(qs_group_plan *g);
 static qs_status refresh_tables(qs_group_plan *g) {
   bool same = g->held.size() == g->plans.size();
   for (size_t i = 0; same && i < g->plans.size(); ++i) same = g->held[i] == storages_of(g->plans[i]);
-  if (same) return QS_OK;
+  if (same && g->tables_ok) return QS_OK;
   if (hipDeviceSynchronize() != hipSuccess) return fail(QS_ERR_DEVICE, "group: sync before rebuilding tables");
   qs_status st = build_tables(g);
   if (st != QS_OK) return st;

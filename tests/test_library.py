@@ -99,6 +99,14 @@ def test_destroy_null_is_noop():
     assert qs._lib.lib().qs_plan_destroy(None) == 0
 
 
+def test_fault_inject_arms_and_rejects_unknown_kinds():
+    lib = qs._lib.lib()
+    assert lib.qs_fault_inject(7, 0) == qs._lib.QS_ERR_ARGUMENT
+    for kind in (qs._lib.QS_FAULT_DEVICE_ALLOC, qs._lib.QS_FAULT_PINNED_ALLOC):
+        assert lib.qs_fault_inject(kind, 3) == 0
+        assert lib.qs_fault_inject(kind, -1) == 0   # disarmed again
+
+
 def test_autotune_cache_file_roundtrip(tmp_path):
     qs.autotune_clear()
     p = tmp_path / "tune.txt"
