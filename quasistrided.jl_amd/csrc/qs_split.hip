@@ -325,14 +325,15 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(CrtOperand o, const Cr
   }
 }
 
-// The residue GEMMs: one 256 x 256 tile of one modulus per workgroup, 8 waves
-// (2 x 4, each 128 x 64 = 8 x 4 MFMA blocks of v_mfma_i32_16x16x64_i8), K in
-// stages of one fragment (64 bytes).  A's 16 row blocks of a stage (16 KiB)
-// go through LDS, double-buffered, loaded two stages ahead; each wave reads
-// its 4 B fragments straight from global memory into registers one stage
-// ahead (a contiguous 1 KiB per wave in the residue layout, wanted by the 2
-// waves of a column, so L1 absorbs the repeat).  Measured against A and B
-// both through LDS: same time or better, half the LDS.
+// The residue GEMMs, for padded K above kCrtMaxKpUnreduced: one 256 x 256
+// tile of one modulus per workgroup, 8 waves (2 x 4, each 128 x 64 = 8 x 4
+// MFMA blocks of v_mfma_i32_16x16x64_i8), K in stages of one fragment (64
+// bytes).  A's 16 row blocks of a stage (16 KiB) go through LDS,
+// double-buffered, loaded two stages ahead; each wave reads its 4 B
+// fragments straight from global memory into registers one stage ahead (a
+// contiguous 1 KiB per wave in the residue layout, wanted by the 2 waves of a
+// column, so L1 absorbs the repeat).  The INT32 accumulators are reduced mod
+// m every 4096 K.
 struct CrtGemmArgs {
   const int8_t *ra, *rb;
   unsigned int *u;           // [Mp/16][Np/16][n][64]
@@ -342,7 +343,13 @@ struct CrtGemmArgs {
   int xcds;
   int m[kMaxModuli];
   float invm[kMaxModuli];
+  double invmd[kMaxModuli];
 };
+
+// Longest padded K the four-wave kernel takes: every INT32 sum of Kp
+// products of two residues (|r| <= 128) stays below 2^31, so no reduction
+// is needed inside the K loop.
+static constexpr int64_t kCrtMaxKpUnreduced = 130816;
 
 __device__ __forceinline__ int crt_mod(int x, int m, float invm) {
   // |x| < 2^27: the float quotient is off by at most one.
@@ -353,11 +360,12 @@ __device__ __forceinline__ int crt_mod(int x, int m, float invm) {
   return r;
 }
 
-// WM x WN waves, each TI x TJ blocks of 16 x 16: a (16 WM TI) x (16 WN TJ)
-// tile of one modulus per workgroup.  B is loaded BS - 1 stages ahead into
-// BS rotating register sets (2 measured ahead of 4).
+// Padded K above kCrtMaxKpUnreduced.  WM x WN waves, each TI x TJ blocks of
+// 16 x 16: a (16 WM TI) x (16 WN TJ) tile of one modulus per workgroup.  B is
+// loaded BS - 1 stages ahead into BS rotating register sets (2 measured ahead
+// of 4).
 template <int WM, int WN, int TI, int TJ, int BS>
-__global__ __launch_bounds__(64 * WM * WN, 1) void crt_gemm_kernel(const CrtGemmArgs a) {
+__global__ __launch_bounds__(64 * WM * WN, 1) void crt_gemm_longk_kernel(const CrtGemmArgs a) {
   constexpr int NW = WM * WN, FA = WM * TI, NLA = FA / NW;
   static_assert(FA % NW == 0, "A stage slots must split evenly over the waves");
   static_assert(BS == 2 || BS == 4, "B register sets: 2 or 4 (the loop is unrolled by 4)");
@@ -469,6 +477,134 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void crt_gemm_kernel(const CrtGemm
       for (int r = 0; r < 4; ++r) w |= (unsigned int)crt_mod(acc[i][j][r], m, invm) << (8 * r);
       const int64_t blk = (mb0 + wm * TI + i) * a.NB + nb0 + wn * TJ + j;
       ((QS_GLOBAL unsigned int *)a.u)[(blk * a.n + l) * 64 + lane] = w;
+    }
+}
+
+typedef __attribute__((address_space(3))) void qs_lds_void;
+
+// One MFMA whose accumulator is tied to an AGPR quad: the compiler otherwise
+// rotates 256 accumulator registers through copies every iteration.  Issued
+// in program order; the loads between them are placed by hand.
+__device__ __forceinline__ void mfma_i8_agpr(qs_v4i &acc, const qs_v4i &a, const qs_v4i &b) {
+  asm volatile("v_mfma_i32_16x16x64_i8 %0, %1, %2, %0" : "+a"(acc) : "v"(a), "v"(b));
+}
+
+// The residue GEMMs (padded K up to kCrtMaxKpUnreduced): one 256 x 256 tile
+// of one modulus per workgroup, 4 waves (2 x 2) of 128 x 128 = 8 x 8 MFMA
+// blocks, one wave per SIMD with its 256 accumulators in AGPRs.  K runs in
+// stages of one 64-byte fragment.  A stage's 16 A and 16 B fragment blocks
+// (32 KiB) are copied by buffer loads straight into LDS (4 + 4 blocks per
+// wave), three stages ahead, through a ring of four buffers; the fragments of
+// stage kb + 1 are read from LDS during the first half of stage kb's 64
+// MFMAs, into the second of two register sets.  One barrier per stage.  The
+// epilogue reduces each sum mod m exactly in FP64 (x / m is never within
+// 2^-20 below an integer unless it is one, for |x| < 2^31) and stores the
+// bytes.  Against eight waves of 128 x 64 with B read straight into
+// registers: 32 against 48 KiB of vector-memory reads and 96 against 80 KiB
+// of LDS traffic per CU and stage, 8-9 % faster at M = N = K = 4096.
+__global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const CrtGemmArgs a) {
+  constexpr int TI = 8, TJ = 8, SB = 32 * 1024;
+  __shared__ __attribute__((aligned(1024))) int8_t smem[4 * SB];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave & 1, wn = wave >> 1;
+  const int64_t tiles = a.tiles_m * a.tiles_n;
+  const int64_t bid = xcd_swizzle(blockIdx.x, (int64_t)gridDim.x, a.xcds);
+  const int l = (int)(bid / tiles);
+  const int64_t tt = bid % tiles;
+  // groups of 4 tile rows, walked along N
+  const int64_t grp = tt / (4 * a.tiles_n), in = tt % (4 * a.tiles_n);
+  const int64_t rows = min((int64_t)4, a.tiles_m - 4 * grp);
+  const int64_t mb0 = (4 * grp + in % rows) * 16, nb0 = (in / rows) * 16;
+  const int KB = (int)a.KB;
+  // the tile's 16 row blocks of one operand over all of K: contiguous in the residue layout
+  const int panel = 16 * KB * 1024;
+  const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(a.ra + ((int64_t)l * a.MB + mb0) * (int64_t)KB * 1024), (short)0, panel, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(a.rb + ((int64_t)l * a.NB + nb0) * (int64_t)KB * 1024), (short)0, panel, 0x00020000);
+  const unsigned vo = (unsigned)lane * 16u;
+  const double md = (double)a.m[l], imd = a.invmd[l];
+
+  // copy q (0..7) of stage kb into LDS buffer buf: block wave + 4 (q / 2) of A (q even) or B (q odd)
+  auto dcopy = [&](int kb, int buf, int q) {
+    const int blk = wave + 4 * (q >> 1);
+    if ((q & 1) == 0)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (qs_lds_void *)(smem + buf * SB + blk * 1024), 16, vo,
+                                               (blk * KB + kb) * 1024, 0, 0);
+    else
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (qs_lds_void *)(smem + buf * SB + (16 + blk) * 1024), 16, vo,
+                                               (blk * KB + kb) * 1024, 0, 0);
+  };
+  auto rdA = [&](int buf, int i) { return *(const qs_v4i *)(smem + buf * SB + (wm * TI + i) * 1024 + vo); };
+  auto rdB = [&](int buf, int j) { return *(const qs_v4i *)(smem + buf * SB + (16 + wn * TJ + j) * 1024 + vo); };
+
+  qs_v4i acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = qs_v4i{0, 0, 0, 0};
+  qs_v4i af[2][TI], bf[2][TJ];
+
+  // Stage kb (u = kb & 1): multiply register set u; copy stage kb + 3 (the
+  // last stage again past the end) into buffer (kb + 3) & 3, which held stage
+  // kb - 1, read during stage kb - 2; read stage kb + 1 from buffer
+  // (kb + 1) & 3 into set u ^ 1.
+  auto step = [&](int kb, auto U) {
+    constexpr int u = decltype(U)::value;
+    const int nb = (kb + 1) & 3, cb = (kb + 3) & 3;
+    const int kc = kb + 3 < KB ? kb + 3 : KB - 1;
+#pragma unroll
+    for (int i = 0; i < TI; ++i) {
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) {
+        mfma_i8_agpr(acc[i][j], af[u][i], bf[u][j]);
+        if (j == 1) dcopy(kc, cb, i);
+        if (i < 4) {
+          if (j == 2) bf[u ^ 1][2 * i] = rdB(nb, 2 * i);
+          if (j == 4) bf[u ^ 1][2 * i + 1] = rdB(nb, 2 * i + 1);
+          if (j == 6) af[u ^ 1][2 * i] = rdA(nb, 2 * i);
+          if (j == 7) af[u ^ 1][2 * i + 1] = rdA(nb, 2 * i + 1);
+        }
+      }
+    }
+    // Stage kb + 2 landed in every wave (stage kb + 3's 8 copies may still be
+    // in flight; vector memory counts retire in order), and every wave's
+    // fragment reads are done before any buffer is refilled.
+    asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  };
+
+  for (int q = 0; q < 8; ++q) dcopy(0, 0, q);
+  for (int q = 0; q < 8; ++q) dcopy(1 < KB ? 1 : KB - 1, 1, q);
+  for (int q = 0; q < 8; ++q) dcopy(2 < KB ? 2 : KB - 1, 2, q);
+  asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory");
+#pragma unroll
+  for (int i = 0; i < TI; ++i) {
+    af[0][i] = rdA(0, i);
+    bf[0][i] = rdB(0, i);
+  }
+  // stage 1 landed; the accumulators' zeroing writes settled before the first MFMA reads them
+  asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier\n\ts_nop 7\n\ts_nop 7" ::: "memory");
+  for (int kb = 0; kb < KB; kb += 2) {   // KB is a multiple of 4
+    step(kb, std::integral_constant<int, 0>());
+    step(kb + 1, std::integral_constant<int, 1>());
+  }
+  // No copy outstanding at exit; the last MFMAs' passes done before the AGPRs are read.
+  asm volatile("s_waitcnt vmcnt(0)\n\ts_nop 15\n\ts_nop 15\n\ts_nop 15\n\ts_nop 15" ::: "memory");
+  QS_GLOBAL unsigned int *U = (QS_GLOBAL unsigned int *)a.u;
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+      unsigned int w = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double x = (double)acc[i][j][r];
+        const double q = floor(__builtin_fma(x, imd, 0x1p-20));
+        w |= (unsigned int)__builtin_fma(-q, md, x) << (8 * r);
+      }
+      const int64_t blk = (mb0 + wm * TI + i) * a.NB + nb0 + wn * TJ + j;
+      U[(blk * a.n + l) * 64 + lane] = w;
     }
 }
 
@@ -846,13 +982,19 @@ static hipError_t split_compute(qs_plan *p, const GemmArgs &g, hipStream_t s) {
   for (int l = 0; l < kMaxModuli; ++l) {
     ga.m[l] = l < n ? split_moduli_table()[l] : 1;
     ga.invm[l] = 1.0f / (float)ga.m[l];
+    ga.invmd[l] = 1.0 / (double)ga.m[l];
   }
-  // 256 x 256 tiles, 8 waves of 128 x 64, one workgroup per CU: measured
-  // ahead of 128 x 128 (4 waves, 3 per CU) and 256 x 128 (8 waves, 2 per CU).
+  // 256 x 256 tiles, one workgroup per CU.  Up to kCrtMaxKpUnreduced: four
+  // waves of 128 x 128 fed through LDS by buffer-to-LDS copies.  Beyond it,
+  // eight waves of 128 x 64 with mid-K reductions (measured ahead of 128 x 128
+  // tiles with 4 waves, 3 per CU, and 256 x 128 with 8 waves, 2 per CU).
   ga.tiles_m = p->split_Mp / kTile;
   ga.tiles_n = p->split_Np / kTile;
-  hipLaunchKernelGGL((crt_gemm_kernel<2, 4, 8, 4, 2>), dim3((unsigned)(ga.tiles_m * ga.tiles_n * n)), dim3(512), 0,
-                     s, ga);
+  const dim3 ggrid((unsigned)(ga.tiles_m * ga.tiles_n * n));
+  if (p->split_Kp <= kCrtMaxKpUnreduced)
+    hipLaunchKernelGGL(crt_gemm_kernel, ggrid, dim3(256), 0, s, ga);
+  else
+    hipLaunchKernelGGL((crt_gemm_longk_kernel<2, 4, 8, 4, 2>), ggrid, dim3(512), 0, s, ga);
 
   CrtRecArgs ra;
   ra.g = g;

@@ -121,10 +121,13 @@ def _plain_pair(rng, M, K, N, row_exp=None):
     return A, B, Av, Bv, Cv
 
 
-def test_split_long_k_reduces_accumulators(oracle):
-    case("split F64, K = 9000 (> 4096: INT32 accumulators reduced mod m mid-K)")
+@pytest.mark.parametrize("M,K,N", [(70, 9000, 40), (24, 140000, 20)])
+def test_split_long_k(oracle, M, K, N):
+    """K = 9000: INT32 sums up to 2^27 on the four-wave kernel (no reduction
+    inside its K loop up to a padded K of 130816).  K = 140000: past that, the
+    eight-wave kernel, whose accumulators are reduced mod m every 4096 K."""
+    case(f"split F64, M={M} K={K} N={N}")
     rng = np.random.default_rng(11)
-    M, K, N = 70, 9000, 40
     A, B, Av, Bv, Cv = _plain_pair(rng, M, K, N)
     c0 = storage_np(Cv.parent)
     plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split")
