@@ -119,14 +119,16 @@ __device__ __forceinline__ int ssq_expo(double x) {
 // rescaling), and lanes and waves are combined by fixed butterflies, so the
 // scales, hence the results, are reproducible bit for bit.  Lanes outside
 // the operand read its first element and discard it.
-__global__ __launch_bounds__(512) void crt_norm_kernel(CrtOperand o) {
-  __shared__ Ssq red[8][64];
+template <typename T>
+__device__ __forceinline__ void crt_norm_body(const CrtOperand &o, const T *__restrict__ X,
+                                              const int64_t *__restrict__ rowtab, const int64_t *__restrict__ ktab,
+                                              Ssq *__restrict__ part, Ssq (&red)[8][64], int64_t bx) {
+  const QS_GLOBAL T *Xg = (const QS_GLOBAL T *)X;
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  const int64_t r0 = bx * 64;
   const int64_t kc0 = (int64_t)blockIdx.y * kNormChunk;
-  const int64_t safe = o.base + o.rowtab[0] + o.ktab[0];
-  double im;
+  const int64_t safe = o.base + rowtab[0] + ktab[0];
   if (o.kfast) {
     // Lanes along K (k = kc0 + 64 j + lane), rows wave + 8 it of the group.
     int64_t ko[4];
@@ -135,18 +137,18 @@ __global__ __launch_bounds__(512) void crt_norm_kernel(CrtOperand o) {
     for (int j = 0; j < 4; ++j) {
       const int64_t k = kc0 + 64 * j + lane;
       kin[j] = k < o.K;
-      ko[j] = o.ktab[kin[j] ? k : 0];
+      ko[j] = ktab[kin[j] ? k : 0];
     }
     double x[8][4];
 #pragma unroll
     for (int it = 0; it < 8; ++it) {
       const int64_t r = r0 + wave + 8 * it;
       const bool rin = r < o.R;
-      const int64_t rb = o.base + o.rowtab[rin ? r : 0];
+      const int64_t rb = o.base + rowtab[rin ? r : 0];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const bool ok = rin && kin[j];
-        load_elem<double, false, true>(o.X, o.type, ok ? rb + ko[j] : safe, x[it][j], im);
+        x[it][j] = (double)Xg[ok ? rb + ko[j] : safe];
         x[it][j] = ok ? x[it][j] : 0.0;
       }
     }
@@ -164,19 +166,19 @@ __global__ __launch_bounds__(512) void crt_norm_kernel(CrtOperand o) {
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) sq += __shfl_xor(sq, off, 64);   // same sum in every lane
       const int64_t r = r0 + wave + 8 * it;
-      if (lane == 0 && r < o.R) o.part[(int64_t)blockIdx.y * o.R + r] = Ssq{sq, e, 0};
+      if (lane == 0 && r < o.R) part[(int64_t)blockIdx.y * o.R + r] = Ssq{sq, e, 0};
     }
   } else {
     // Lanes along the rows (row r0 + lane), K steps wave + 8 i.
     const int64_t r = r0 + lane;
     const bool rin = r < o.R;
-    const int64_t rbase = o.base + o.rowtab[rin ? r : 0];
+    const int64_t rbase = o.base + rowtab[rin ? r : 0];
     double x[32];
 #pragma unroll
     for (int i = 0; i < 32; ++i) {
       const int64_t k = kc0 + wave + 8 * i;
       const bool ok = rin && k < o.K;
-      load_elem<double, false, true>(o.X, o.type, ok ? rbase + o.ktab[ok ? k : 0] : safe, x[i], im);
+      x[i] = (double)Xg[ok ? rbase + ktab[ok ? k : 0] : safe];
       x[i] = ok ? x[i] : 0.0;
     }
     int ex[32];
@@ -199,24 +201,22 @@ __global__ __launch_bounds__(512) void crt_norm_kernel(CrtOperand o) {
       Ssq a = red[0][t];
 #pragma unroll
       for (int w = 1; w < 8; ++w) a = ssq_merge(a, red[w][t]);
-      if (r0 + t < o.R) o.part[(int64_t)blockIdx.y * o.R + r0 + t] = a;
+      if (r0 + t < o.R) part[(int64_t)blockIdx.y * o.R + r0 + t] = a;
     }
   }
 }
 
-// Pass 1b: one thread per row merges the row's partials in chunk order into
-// its scale: q = T / f and 2^sh with norm = f 2^(e + en0).
-__global__ __launch_bounds__(256) void crt_scale_kernel(CrtOperand o) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (r >= o.R) return;
-  Ssq a = Ssq{0.0, -2000, 0};
-  for (int c0 = 0; c0 < o.nkc; c0 += 8) {
-    Ssq b[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) b[c] = c0 + c < o.nkc ? o.part[(int64_t)(c0 + c) * o.R + r] : Ssq{0.0, -2000, 0};
-#pragma unroll
-    for (int c = 0; c < 8; ++c) a = ssq_merge(a, b[c]);
-  }
+// T: the operand's storage type (real: split precision refuses complex
+// compute types).
+template <typename T>
+__global__ __launch_bounds__(512) void crt_norm_kernel(const CrtOperand o) {
+  __shared__ Ssq red[8][64];
+  crt_norm_body<T>(o, (const T *)o.X, o.rowtab, o.ktab, o.part, red, blockIdx.x);
+}
+
+// The row's scale from its norm (scaled sum of squares): q = T / f and 2^sh
+// with norm = f 2^(e + en0).
+__device__ __forceinline__ RowScale row_scale_of(const CrtOperand &o, const Ssq a) {
   RowScale rs{0.0, 1.0, 0, 0};
   if (!(a.s <= DBL_MAX)) {
     rs.iq = __builtin_nan("");
@@ -227,48 +227,172 @@ __global__ __launch_bounds__(256) void crt_scale_kernel(CrtOperand o) {
     rs.iq = f / o.Tm;
     rs.sh = (en0 + a.e) - o.eT;
   }
-  o.scale[r] = rs;
+  return rs;
+}
+
+// Pass 1b: one thread per row merges the row's partials in chunk order into
+// its scale: q = T / f and 2^sh with norm = f 2^(e + en0).
+__global__ __launch_bounds__(256) void crt_scale_kernel(const CrtOperand o) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= o.R) return;
+  Ssq a = Ssq{0.0, -2000, 0};
+  for (int c0 = 0; c0 < o.nkc; c0 += 8) {
+    Ssq b[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) b[c] = c0 + c < o.nkc ? o.part[(int64_t)(c0 + c) * o.R + r] : Ssq{0.0, -2000, 0};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) a = ssq_merge(a, b[c]);
+  }
+  o.scale[r] = row_scale_of(o, a);
+}
+
+// The row norms and scales in one pass (no partial sums, no separate merge):
+// each workgroup of four waves takes a few rows over all of K, with its loads
+// batched ahead of their arithmetic, and writes the rows' scales itself.
+// Each batch's squares are scaled by the batch's largest exponent (per lane)
+// and merged into the lane's running sum; lanes and waves are merged by fixed
+// trees, so the scales are reproducible bit for bit.
+//   K fast: one row per wave (4 per workgroup), lanes along K, 32 K steps of
+//           64 per batch.
+//   Otherwise: 16 rows per workgroup; lanes 16 rows x 4 K offsets (a load
+//           covers 4 runs of 16 elements along the rows), wave w takes K steps
+//           4 w + 16 i, 32 per batch.
+constexpr int kRowsKFast = 4, kRowsMFast = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void crt_rownorm_kernel(const CrtOperand o) {
+  constexpr int NB = 32;
+  __shared__ Ssq red[4][16];
+  const QS_GLOBAL T *X = (const QS_GLOBAL T *)o.X;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int64_t safe = o.base + o.rowtab[0] + o.ktab[0];
+  auto batch = [](const double (&x)[NB], Ssq &acc) {
+    int e = -2000;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) e = max(e, ssq_expo(x[j]));
+    double sq = 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const double y = ldexp(fabs(x[j]), -e);
+      sq = __builtin_fma(y, y, sq);
+    }
+    acc = ssq_merge(acc, Ssq{sq, e, 0});
+  };
+  Ssq acc{0.0, -2000, 0};
+  if (o.kfast) {
+    const int64_t r = (int64_t)blockIdx.x * kRowsKFast + wave;
+    const bool rin = r < o.R;
+    const int64_t rb = o.base + o.rowtab[rin ? r : 0];
+    for (int64_t k0 = 0; k0 < o.K; k0 += 64 * NB) {
+      int64_t ko[NB];
+      bool kin[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int64_t k = k0 + 64 * j + lane;
+        kin[j] = k < o.K;
+        ko[j] = o.ktab[kin[j] ? k : 0];
+      }
+      double x[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) x[j] = (double)X[rin && kin[j] ? rb + ko[j] : safe];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) x[j] = rin && kin[j] ? x[j] : 0.0;
+      batch(x, acc);
+    }
+#pragma unroll
+    for (int off = 1; off <= 32; off <<= 1) {
+      const Ssq b{__shfl_xor(acc.s, off, 64), __shfl_xor(acc.e, off, 64), 0};
+      acc = ssq_merge(acc, b);   // commutative: both lanes of a pair form the same value
+    }
+    if (lane == 0 && rin) o.scale[r] = row_scale_of(o, acc);
+  } else {
+    const int64_t r0 = (int64_t)blockIdx.x * kRowsMFast;
+    const int rr = lane & 15, ks = lane >> 4;
+    const int64_t r = r0 + rr;
+    const bool rin = r < o.R;
+    const int64_t rbase = o.base + o.rowtab[rin ? r : 0];
+    for (int64_t k0 = 0; k0 < o.K; k0 += 16 * NB) {
+      int64_t ko[NB];
+      bool kin[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int64_t k = k0 + 16 * j + 4 * wave + ks;
+        kin[j] = k < o.K;
+        ko[j] = o.ktab[kin[j] ? k : 0];
+      }
+      double x[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) x[j] = (double)X[rin && kin[j] ? rbase + ko[j] : safe];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) x[j] = rin && kin[j] ? x[j] : 0.0;
+      batch(x, acc);
+    }
+    // lanes rr, rr + 16, rr + 32, rr + 48 hold one row
+#pragma unroll
+    for (int off = 16; off <= 32; off <<= 1) {
+      const Ssq b{__shfl_xor(acc.s, off, 64), __shfl_xor(acc.e, off, 64), 0};
+      acc = ssq_merge(acc, b);
+    }
+    if (lane < 16) red[wave][lane] = acc;
+    __syncthreads();
+    if (t < 16) {
+      Ssq a = red[0][t];
+#pragma unroll
+      for (int w = 1; w < 4; ++w) a = ssq_merge(a, red[w][t]);
+      if (r0 + t < o.R) o.scale[r0 + t] = row_scale_of(o, a);
+    }
+  }
 }
 
 // Pass 2: scale, round and reduce one 64-row x 64-K tile to one residue
 // fragment column per modulus.
-__global__ __launch_bounds__(256) void crt_residue_kernel(CrtOperand o, const CrtConst c) {
-  __shared__ double tile[64][65];
-  __shared__ double sq[64];
-  __shared__ int ssh[64];
+struct CrtResidueLds {
+  double tile[64][65];
+  double sq[64];
+  int ssh[64];
+};
+
+template <typename T>
+__device__ __forceinline__ void crt_residue_body(const CrtOperand &o, const CrtConst &c, const T *__restrict__ X,
+                                                 const int64_t *__restrict__ rowtab, const int64_t *__restrict__ ktab,
+                                                 const RowScale *__restrict__ scale, int8_t *__restrict__ res,
+                                                 CrtResidueLds &sm, int64_t bx) {
+  const QS_GLOBAL T *Xg = (const QS_GLOBAL T *)X;
+  auto &tile = sm.tile;
+  auto &sq = sm.sq;
+  auto &ssh = sm.ssh;
   const int t = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
+  const int64_t r0 = bx * 64, k0 = (int64_t)blockIdx.y * 64;
   // 16 independent loads per thread, issued before the row scales are read.
   double x[16];
   if (o.kfast) {
     // Lanes along K (tk = lane), rows wave + 4 it.
     const int64_t k = k0 + (t & 63);
     const bool kin = k < o.K;
-    const int64_t ko = kin ? o.ktab[k] : 0;
+    const int64_t ko = kin ? ktab[k] : 0;
 #pragma unroll
     for (int it = 0; it < 16; ++it) {
       const int64_t r = r0 + wave + 4 * it;
-      double im;
       x[it] = 0.0;
-      if (kin && r < o.R) load_elem<double, false, true>(o.X, o.type, o.base + o.rowtab[r] + ko, x[it], im);
+      if (kin && r < o.R) x[it] = (double)Xg[o.base + rowtab[r] + ko];
     }
   } else {
     // Lanes along the rows (tr = lane), K steps wave + 4 it.
     const int64_t r = r0 + (t & 63);
     const bool rin = r < o.R;
-    const int64_t rbase = rin ? o.base + o.rowtab[r] : 0;
+    const int64_t rbase = rin ? o.base + rowtab[r] : 0;
 #pragma unroll
     for (int it = 0; it < 16; ++it) {
       const int64_t k = k0 + wave + 4 * it;
-      double im;
       x[it] = 0.0;
-      if (rin && k < o.K) load_elem<double, false, true>(o.X, o.type, rbase + o.ktab[k], x[it], im);
+      if (rin && k < o.K) x[it] = (double)Xg[rbase + ktab[k]];
     }
   }
   if (t < 64) {
     // Rows past R (padding) get q = 0: their integers are zero.
-    const RowScale rs = r0 + t < o.R ? o.scale[r0 + t] : RowScale{0.0, 1.0, 0, 0};
+    const RowScale rs = r0 + t < o.R ? scale[r0 + t] : RowScale{0.0, 1.0, 0, 0};
     sq[t] = rs.q;
     ssh[t] = -rs.sh;
   }
@@ -321,8 +445,14 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(CrtOperand o, const Cr
       word[q] = (int)(packed ^ 0x80808080u);            // two's complement of the residue
     }
     const qs_v4i out = {word[0], word[1], word[2], word[3]};
-    __builtin_nontemporal_store(out, (QS_GLOBAL qs_v4i *)(o.res + ((int64_t)l * RB * KB + frag) * 1024 + lane * 16));
+    __builtin_nontemporal_store(out, (QS_GLOBAL qs_v4i *)(res + ((int64_t)l * RB * KB + frag) * 1024 + lane * 16));
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void crt_residue_kernel(const CrtOperand o, const CrtConst c) {
+  __shared__ CrtResidueLds sm;
+  crt_residue_body<T>(o, c, (const T *)o.X, o.rowtab, o.ktab, o.scale, o.res, sm, blockIdx.x);
 }
 
 // The residue GEMMs, for padded K above kCrtMaxKpUnreduced: one 256 x 256
@@ -614,13 +744,15 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const CrtGemmArgs a) {
 // elements of C.
 struct CrtRecArgs {
   GemmArgs g;
-  const unsigned int *u;
-  const RowScale *sa, *sb;
   int64_t MB, NB;
 };
 
-template <bool MFAST>
-__global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const CrtConst c) {
+// R: the compute type; MIXED: C stored in the other real type.
+template <bool MFAST, typename R, bool MIXED>
+__global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const CrtConst c,
+                                                      const unsigned int *__restrict__ ures,
+                                                      const RowScale *__restrict__ sa, const RowScale *__restrict__ sb,
+                                                      const int64_t *__restrict__ mC, const int64_t *__restrict__ nC) {
   __shared__ double tile[4][MFAST ? 64 : 16][MFAST ? 17 : 65];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t strip = (int64_t)blockIdx.x * 4 + wave;
@@ -633,7 +765,7 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
     nb0 = 4 * (strip / a.MB);
   }
   const GemmArgs &g = a.g;
-  const QS_GLOBAL unsigned int *u = (const QS_GLOBAL unsigned int *)a.u;
+  const QS_GLOBAL unsigned int *u = (const QS_GLOBAL unsigned int *)ures;
   // The residue words of RB blocks are loaded before any arithmetic on them.
   constexpr int RB = 2;   // four measured slower (102 against 95 us at 4096 x 4096, 13 moduli)
 #pragma unroll 1
@@ -678,39 +810,29 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
   // old-C loads are issued before the first store.
   const int64_t fixed = MFAST ? mb0 * 16 + lane : nb0 * 16 + lane;
   if (fixed >= (MFAST ? g.M : g.N)) return;
-  const RowScale fs = MFAST ? a.sa[fixed] : a.sb[fixed];
-  const int64_t foff = MFAST ? g.mC[fixed] : g.nC[fixed];
+  const RowScale fs = MFAST ? sa[fixed] : sb[fixed];
+  const int64_t foff = MFAST ? mC[fixed] : nC[fixed];
 #pragma unroll
   for (int h = 0; h < 16; h += 8) {
     int64_t idx[8];
     bool ok[8];
-    double v[8], o[8];
+    double v[8];
+    R o[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int e = h + j;
       const int64_t other = (MFAST ? nb0 : mb0) * 16 + e;
       ok[j] = other < (MFAST ? g.N : g.M);
-      const RowScale os = ok[j] ? (MFAST ? a.sb[other] : a.sa[other]) : RowScale{0.0, 1.0, 0, 0};
-      idx[j] = ok[j] ? g.c_base + foff + (MFAST ? g.nC[other] : g.mC[other]) : 0;
+      const RowScale os = ok[j] ? (MFAST ? sb[other] : sa[other]) : RowScale{0.0, 1.0, 0, 0};
+      idx[j] = ok[j] ? g.c_base + foff + (MFAST ? nC[other] : mC[other]) : 0;
       v[j] = ldexp((MFAST ? tile[wave][lane][e] : tile[wave][e][lane]) * fs.iq * os.iq, fs.sh + os.sh);
-      o[j] = 0.0;
-      if (ok[j]) {
-        if (g.compute_type == QS_F64) {
-          double oi;
-          load_c_old<double, false, true>(g, idx[j], o[j], oi);
-        } else {
-          float of, ofi;
-          load_c_old<float, false, true>(g, idx[j], of, ofi);
-          o[j] = of;
-        }
-      }
+      o[j] = 0;
+      R oi;
+      if (ok[j]) load_c_old<R, false, MIXED>(g, idx[j], o[j], oi);
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      if (!ok[j]) continue;
-      if (g.compute_type == QS_F64) store_c_at<double, false, true, true>(g, idx[j], v[j], 0.0, o[j], 0.0);
-      else store_c_at<float, false, true, true>(g, idx[j], (float)v[j], 0.0f, (float)o[j], 0.0f);
-    }
+    for (int j = 0; j < 8; ++j)
+      if (ok[j]) store_c_at<R, false, MIXED, true>(g, idx[j], (R)v[j], (R)0, o[j], (R)0);
   }
 }
 
@@ -954,17 +1076,24 @@ static hipError_t split_compute(qs_plan *p, const GemmArgs &g, hipStream_t s) {
   ob.X = g.B; ob.type = g.b_type; ob.kfast = p->b_kfast; ob.base = g.b_base; ob.rowtab = g.nB; ob.ktab = g.kB;
   ob.R = p->N; ob.Rp = p->split_Np; ob.side = 1;
   ob.part = (Ssq *)(base + L.partB); ob.scale = (RowScale *)(base + L.scB); ob.res = (int8_t *)(base + L.rb);
-  // A's passes on the caller's stream, B's on the side stream: both are
-  // HBM-bound, and side by side their ramps and tails overlap.
+  // A's passes on the caller's stream, B's on the side stream, forked and
+  // joined by events.  Measured against one launch per pass covering both
+  // operands (M = N = K = 4096): 226 us before the GEMM against 240 us with
+  // the operands' workgroups in turn and 276 us with them interleaved.
   hipError_t fe = hipEventRecord(p->split_fork, s);
   if (fe == hipSuccess) fe = hipStreamWaitEvent(p->split_side, p->split_fork, 0);
   if (fe != hipSuccess) return fe;
   for (int side = 0; side < 2; ++side) {
     const CrtOperand &o = side ? ob : oa;
     hipStream_t os = side ? p->split_side : s;
-    hipLaunchKernelGGL(crt_norm_kernel, dim3((unsigned)((o.R + 63) / 64), (unsigned)nkc), dim3(512), 0, os, o);
-    hipLaunchKernelGGL(crt_scale_kernel, dim3((unsigned)((o.R + 255) / 256)), dim3(256), 0, os, o);
-    hipLaunchKernelGGL(crt_residue_kernel, dim3((unsigned)(o.Rp / 64), (unsigned)(o.Kp / 64)), dim3(256), 0, os, o, c);
+    const dim3 ngrid((unsigned)((o.R + 63) / 64), (unsigned)nkc), rgrid((unsigned)(o.Rp / 64), (unsigned)(o.Kp / 64));
+    const int rpw = o.kfast ? kRowsKFast : kRowsMFast;
+    const dim3 sgrid((unsigned)((o.R + rpw - 1) / rpw));
+    if (o.type == QS_F32) hipLaunchKernelGGL(crt_rownorm_kernel<float>, sgrid, dim3(256), 0, os, o);
+    else hipLaunchKernelGGL(crt_rownorm_kernel<double>, sgrid, dim3(256), 0, os, o);
+    (void)ngrid;
+    if (o.type == QS_F32) hipLaunchKernelGGL(crt_residue_kernel<float>, rgrid, dim3(256), 0, os, o, c);
+    else hipLaunchKernelGGL(crt_residue_kernel<double>, rgrid, dim3(256), 0, os, o, c);
   }
   fe = hipEventRecord(p->split_join, p->split_side);
   if (fe == hipSuccess) fe = hipStreamWaitEvent(s, p->split_join, 0);
@@ -998,15 +1127,21 @@ static hipError_t split_compute(qs_plan *p, const GemmArgs &g, hipStream_t s) {
 
   CrtRecArgs ra;
   ra.g = g;
-  ra.u = ga.u;
-  ra.sa = oa.scale;
-  ra.sb = ob.scale;
   ra.MB = ga.MB;
   ra.NB = ga.NB;
   // Strips of four blocks, four per workgroup: Mp and Np are multiples of 256.
   const dim3 rgrid((unsigned)(ga.MB * ga.NB / 16));
-  if (p->c_mfast) hipLaunchKernelGGL(crt_rec_kernel<true>, rgrid, dim3(256), 0, s, ra, c);
-  else hipLaunchKernelGGL(crt_rec_kernel<false>, rgrid, dim3(256), 0, s, ra, c);
+  const bool f64 = g.compute_type == QS_F64, mixed = g.c_type != g.compute_type;
+#define QS_REC_LAUNCH(MF, R, MX) \
+  hipLaunchKernelGGL((crt_rec_kernel<MF, R, MX>), rgrid, dim3(256), 0, s, ra, c, ga.u, oa.scale, ob.scale, g.mC, g.nC)
+  if (p->c_mfast) {
+    if (f64) { if (mixed) QS_REC_LAUNCH(true, double, true); else QS_REC_LAUNCH(true, double, false); }
+    else { if (mixed) QS_REC_LAUNCH(true, float, true); else QS_REC_LAUNCH(true, float, false); }
+  } else {
+    if (f64) { if (mixed) QS_REC_LAUNCH(false, double, true); else QS_REC_LAUNCH(false, double, false); }
+    else { if (mixed) QS_REC_LAUNCH(false, float, true); else QS_REC_LAUNCH(false, float, false); }
+  }
+#undef QS_REC_LAUNCH
   return hipGetLastError();
 }
 
