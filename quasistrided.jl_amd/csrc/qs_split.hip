@@ -19,7 +19,8 @@
 //      stored in MFMA fragment order: 16 rows x 64 K per 1 KiB block, lane l
 //      holding row l % 16, K bytes 16 (l / 16) .. +15.
 //   3. n INT8 GEMMs on v_mfma_i32_16x16x64_i8 with exact INT32 accumulation
-//      (reduced mod m every 4096 K, so K is unbounded); the epilogue stores
+//      (up to a padded K of 130816 the sums stay below 2^31; beyond it they
+//      are reduced mod m every 4096 K, so K is unbounded); the epilogue stores
 //      U_l = acc mod m_l as a byte.
 //   4. Reconstruction.  C' = sum_l U_l Q_l - q P with Q_l = P / m_l and
 //      q = rint(sum_l U_l Q_l / P).  Q_l = Q1_l + Q2_l where Q1_l is a
@@ -48,7 +49,6 @@ namespace qs {
 typedef int qs_v4i __attribute__((ext_vector_type(4)));
 
 static constexpr int kMaxModuli = 15;    // (n + 1) P < 2^128
-static constexpr int kNormChunk = 256;   // K per partial sum of squares
 static constexpr int kTile = 256;        // GEMM workgroup tile (M and N)
 
 // Scaled sum of squares: value s * 2^(2e).
@@ -80,25 +80,12 @@ struct CrtOperand {
   int64_t base;
   const int64_t *rowtab, *ktab;
   int64_t R, K, Rp, Kp;   // rows (M of A, N of B), K, padded to 256 and 128
-  int nkc;                // K chunks of the norm pass
   int side;               // 0: A, 1: B
-  Ssq *part;              // [nkc][R]
   RowScale *scale;        // [R]
   int8_t *res;            // [n][Rp/16][Kp/64][1024]
   double Tm;              // T = Tm * 2^eT
   int eT;
 };
-
-// Branch-free: Inf / NaN make s Inf / NaN, which marks the row.
-__device__ __forceinline__ void ssq_add(Ssq &a, double x) {
-  const double ax = fabs(x);
-  const int ex = (ax > 0.0 && ax <= DBL_MAX) ? ilogb(ax) : -2000;
-  const int ne = max(a.e, ex);
-  a.s = ldexp(a.s, 2 * (a.e - ne));
-  a.e = ne;
-  const double y = ldexp(ax, -ne);
-  a.s = __builtin_fma(y, y, a.s);
-}
 
 __device__ __forceinline__ Ssq ssq_merge(Ssq a, Ssq b) {
   Ssq r;
@@ -111,107 +98,6 @@ __device__ __forceinline__ Ssq ssq_merge(Ssq a, Ssq b) {
 __device__ __forceinline__ int ssq_expo(double x) {
   const double ax = fabs(x);
   return (ax > 0.0 && ax <= DBL_MAX) ? ilogb(ax) : -2000;
-}
-
-// Pass 1: partial sums of squares of 64 rows over one K chunk of 256.  Every
-// load of a thread is issued before any arithmetic; each partial is scaled
-// by the largest exponent it holds (one ldexp per element, no serial
-// rescaling), and lanes and waves are combined by fixed butterflies, so the
-// scales, hence the results, are reproducible bit for bit.  Lanes outside
-// the operand read its first element and discard it.
-template <typename T>
-__device__ __forceinline__ void crt_norm_body(const CrtOperand &o, const T *__restrict__ X,
-                                              const int64_t *__restrict__ rowtab, const int64_t *__restrict__ ktab,
-                                              Ssq *__restrict__ part, Ssq (&red)[8][64], int64_t bx) {
-  const QS_GLOBAL T *Xg = (const QS_GLOBAL T *)X;
-  const int t = threadIdx.x, lane = t & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int64_t r0 = bx * 64;
-  const int64_t kc0 = (int64_t)blockIdx.y * kNormChunk;
-  const int64_t safe = o.base + rowtab[0] + ktab[0];
-  if (o.kfast) {
-    // Lanes along K (k = kc0 + 64 j + lane), rows wave + 8 it of the group.
-    int64_t ko[4];
-    bool kin[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t k = kc0 + 64 * j + lane;
-      kin[j] = k < o.K;
-      ko[j] = ktab[kin[j] ? k : 0];
-    }
-    double x[8][4];
-#pragma unroll
-    for (int it = 0; it < 8; ++it) {
-      const int64_t r = r0 + wave + 8 * it;
-      const bool rin = r < o.R;
-      const int64_t rb = o.base + rowtab[rin ? r : 0];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const bool ok = rin && kin[j];
-        x[it][j] = (double)Xg[ok ? rb + ko[j] : safe];
-        x[it][j] = ok ? x[it][j] : 0.0;
-      }
-    }
-#pragma unroll
-    for (int it = 0; it < 8; ++it) {
-      int e = max(max(ssq_expo(x[it][0]), ssq_expo(x[it][1])), max(ssq_expo(x[it][2]), ssq_expo(x[it][3])));
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) e = max(e, __shfl_xor(e, off, 64));
-      double sq = 0.0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const double y = ldexp(fabs(x[it][j]), -e);
-        sq = __builtin_fma(y, y, sq);
-      }
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) sq += __shfl_xor(sq, off, 64);   // same sum in every lane
-      const int64_t r = r0 + wave + 8 * it;
-      if (lane == 0 && r < o.R) part[(int64_t)blockIdx.y * o.R + r] = Ssq{sq, e, 0};
-    }
-  } else {
-    // Lanes along the rows (row r0 + lane), K steps wave + 8 i.
-    const int64_t r = r0 + lane;
-    const bool rin = r < o.R;
-    const int64_t rbase = o.base + rowtab[rin ? r : 0];
-    double x[32];
-#pragma unroll
-    for (int i = 0; i < 32; ++i) {
-      const int64_t k = kc0 + wave + 8 * i;
-      const bool ok = rin && k < o.K;
-      x[i] = (double)Xg[ok ? rbase + ktab[ok ? k : 0] : safe];
-      x[i] = ok ? x[i] : 0.0;
-    }
-    int ex[32];
-#pragma unroll
-    for (int i = 0; i < 32; ++i) ex[i] = ssq_expo(x[i]);
-#pragma unroll
-    for (int w = 16; w >= 1; w >>= 1)
-#pragma unroll
-      for (int i = 0; i < w; ++i) ex[i] = max(ex[i], ex[i + w]);
-    const int e = ex[0];
-    double sq[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int i = 0; i < 32; ++i) {
-      const double y = ldexp(fabs(x[i]), -e);
-      sq[i & 3] = __builtin_fma(y, y, sq[i & 3]);
-    }
-    red[wave][lane] = Ssq{(sq[0] + sq[1]) + (sq[2] + sq[3]), e, 0};
-    __syncthreads();
-    if (t < 64) {
-      Ssq a = red[0][t];
-#pragma unroll
-      for (int w = 1; w < 8; ++w) a = ssq_merge(a, red[w][t]);
-      if (r0 + t < o.R) part[(int64_t)blockIdx.y * o.R + r0 + t] = a;
-    }
-  }
-}
-
-// T: the operand's storage type (real: split precision refuses complex
-// compute types).
-template <typename T>
-__global__ __launch_bounds__(512) void crt_norm_kernel(const CrtOperand o) {
-  __shared__ Ssq red[8][64];
-  crt_norm_body<T>(o, (const T *)o.X, o.rowtab, o.ktab, o.part, red, blockIdx.x);
 }
 
 // The row's scale from its norm (scaled sum of squares): q = T / f and 2^sh
@@ -228,22 +114,6 @@ __device__ __forceinline__ RowScale row_scale_of(const CrtOperand &o, const Ssq 
     rs.sh = (en0 + a.e) - o.eT;
   }
   return rs;
-}
-
-// Pass 1b: one thread per row merges the row's partials in chunk order into
-// its scale: q = T / f and 2^sh with norm = f 2^(e + en0).
-__global__ __launch_bounds__(256) void crt_scale_kernel(const CrtOperand o) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (r >= o.R) return;
-  Ssq a = Ssq{0.0, -2000, 0};
-  for (int c0 = 0; c0 < o.nkc; c0 += 8) {
-    Ssq b[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) b[c] = c0 + c < o.nkc ? o.part[(int64_t)(c0 + c) * o.R + r] : Ssq{0.0, -2000, 0};
-#pragma unroll
-    for (int c = 0; c < 8; ++c) a = ssq_merge(a, b[c]);
-  }
-  o.scale[r] = row_scale_of(o, a);
 }
 
 // The row norms and scales in one pass (no partial sums, no separate merge):
@@ -990,13 +860,12 @@ static CrtConst crt_constants(int n) {
 static int64_t rup(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 struct SplitLayout {
-  size_t ra, rb, u, partA, partB, scA, scB, smp, rep, total;
+  size_t ra, rb, u, scA, scB, smp, rep, total;
 };
 
 static SplitLayout split_layout(const qs_plan *p) {
   SplitLayout L;
   const int n = p->moduli;
-  const int64_t nkc = (p->K + kNormChunk - 1) / kNormChunk;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     const size_t at = off;
@@ -1006,8 +875,6 @@ static SplitLayout split_layout(const qs_plan *p) {
   L.ra = take((size_t)n * p->split_Mp * p->split_Kp);
   L.rb = take((size_t)n * p->split_Np * p->split_Kp);
   L.u = take((size_t)n * p->split_Mp * p->split_Np);
-  L.partA = take(sizeof(Ssq) * nkc * p->M);
-  L.partB = take(sizeof(Ssq) * nkc * p->N);
   L.scA = take(sizeof(RowScale) * p->M);
   L.scB = take(sizeof(RowScale) * p->N);
   L.smp = take(sizeof(SplitSample) * kSplitSamples);
@@ -1066,16 +933,15 @@ static hipError_t split_compute(qs_plan *p, const GemmArgs &g, hipStream_t s) {
   const CrtConst c = crt_constants(n);
   int eT;
   const double Tm = std::frexp(crt_T(n, p->K), &eT);
-  const int nkc = (int)((p->K + kNormChunk - 1) / kNormChunk);
   CrtOperand oa;
   oa.X = g.A; oa.type = g.a_type; oa.kfast = p->a_kfast; oa.base = g.a_base; oa.rowtab = g.mA; oa.ktab = g.kA;
-  oa.R = p->M; oa.K = p->K; oa.Rp = p->split_Mp; oa.Kp = p->split_Kp; oa.nkc = nkc; oa.side = 0;
-  oa.part = (Ssq *)(base + L.partA); oa.scale = (RowScale *)(base + L.scA); oa.res = (int8_t *)(base + L.ra);
+  oa.R = p->M; oa.K = p->K; oa.Rp = p->split_Mp; oa.Kp = p->split_Kp; oa.side = 0;
+  oa.scale = (RowScale *)(base + L.scA); oa.res = (int8_t *)(base + L.ra);
   oa.Tm = Tm; oa.eT = eT;
   CrtOperand ob = oa;
   ob.X = g.B; ob.type = g.b_type; ob.kfast = p->b_kfast; ob.base = g.b_base; ob.rowtab = g.nB; ob.ktab = g.kB;
   ob.R = p->N; ob.Rp = p->split_Np; ob.side = 1;
-  ob.part = (Ssq *)(base + L.partB); ob.scale = (RowScale *)(base + L.scB); ob.res = (int8_t *)(base + L.rb);
+  ob.scale = (RowScale *)(base + L.scB); ob.res = (int8_t *)(base + L.rb);
   // A's passes on the caller's stream, B's on the side stream, forked and
   // joined by events.  Measured against one launch per pass covering both
   // operands (M = N = K = 4096): 226 us before the GEMM against 240 us with
@@ -1086,12 +952,11 @@ static hipError_t split_compute(qs_plan *p, const GemmArgs &g, hipStream_t s) {
   for (int side = 0; side < 2; ++side) {
     const CrtOperand &o = side ? ob : oa;
     hipStream_t os = side ? p->split_side : s;
-    const dim3 ngrid((unsigned)((o.R + 63) / 64), (unsigned)nkc), rgrid((unsigned)(o.Rp / 64), (unsigned)(o.Kp / 64));
+    const dim3 rgrid((unsigned)(o.Rp / 64), (unsigned)(o.Kp / 64));
     const int rpw = o.kfast ? kRowsKFast : kRowsMFast;
     const dim3 sgrid((unsigned)((o.R + rpw - 1) / rpw));
     if (o.type == QS_F32) hipLaunchKernelGGL(crt_rownorm_kernel<float>, sgrid, dim3(256), 0, os, o);
     else hipLaunchKernelGGL(crt_rownorm_kernel<double>, sgrid, dim3(256), 0, os, o);
-    (void)ngrid;
     if (o.type == QS_F32) hipLaunchKernelGGL(crt_residue_kernel<float>, rgrid, dim3(256), 0, os, o, c);
     else hipLaunchKernelGGL(crt_residue_kernel<double>, rgrid, dim3(256), 0, os, o, c);
   }
