@@ -675,7 +675,11 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
     }
   }
   }
-  __syncthreads();
+  // Each wave reads back only its own tile: a wavefront-scope fence orders
+  // its LDS writes before its reads, and the waves run on independently.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   // Lane `lane` finishes 16 elements along C's unit-stride side: all 16
   // old-C loads are issued before the first store.
   const int64_t fixed = MFAST ? mb0 * 16 + lane : nb0 * 16 + lane;
