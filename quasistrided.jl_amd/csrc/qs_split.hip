@@ -41,6 +41,7 @@
 #include <string>
 #include <vector>
 
+#include "qs_crt.h"
 #include "qs_gemm_impl.h"
 #include "qs_plan.h"
 
@@ -498,8 +499,8 @@ __device__ __forceinline__ void mfma_i8_agpr(qs_v4i &acc, const qs_v4i &a, const
 // stage kb + 1 are read from LDS during the first half of stage kb's 64
 // MFMAs, into the second of two register sets.  One barrier per stage.  The
 // epilogue reduces each sum mod m exactly in FP64 (x / m is never within
-// 2^-20 below an integer unless it is one, for |x| < 2^31) and stores the
-// bytes.  Against eight waves of 128 x 64 with B read straight into
+// 2^-20 below an integer unless it is one, for |x| < 2^31: crt_mod_exact)
+// and stores the bytes.  Against eight waves of 128 x 64 with B read straight into
 // registers: 32 against 48 KiB of vector-memory reads and 96 against 80 KiB
 // of LDS traffic per CU and stage, 8-9 % faster at M = N = K = 4096.
 __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const CrtGemmArgs a) {
@@ -598,11 +599,7 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const CrtGemmArgs a) {
     for (int j = 0; j < TJ; ++j) {
       unsigned int w = 0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const double x = (double)acc[i][j][r];
-        const double q = floor(__builtin_fma(x, imd, 0x1p-20));
-        w |= (unsigned int)__builtin_fma(-q, md, x) << (8 * r);
-      }
+      for (int r = 0; r < 4; ++r) w |= crt_mod_exact(acc[i][j][r], md, imd) << (8 * r);
       const int64_t blk = (mb0 + wm * TI + i) * a.NB + nb0 + wn * TJ + j;
       U[(blk * a.n + l) * 64 + lane] = w;
     }
