@@ -85,11 +85,13 @@ def config4(qs, torch, dev):
     return plan, 2 * 4096 ** 3, 3 * 8 * 4096 ** 2, (A, B, C), 0.0
 
 
-def bench_split(qs, torch, dev, d=64, reps=10):
-    """Rank-4 all-64 F64 with entries N(0,1) * 2^u (u integer in [-20, 20] per
-    row of A and per column of B): autotuned plain FP64 MFMA vs split precision (INT8
-    residue GEMMs), both timed per execute, sustained and interleaved; the split
-    error from the sampled Dot2 report."""
+def bench_split(qs, torch, dev, d=64, reps=10, dtype=None):
+    """Rank-4 all-64 with entries N(0,1) * 2^u (u integer in [-20, 20] per
+    row of A and per column of B), Float64 (or Float32): autotuned plain MFMA
+    of the type vs split precision (INT8 residue GEMMs; target 1e-13 for
+    Float64, 1e-6 for Float32), both timed per execute, sustained and
+    interleaved; the split error from the sampled Dot2 report."""
+    dtype = dtype or torch.float64
     rng = np.random.default_rng(4)
     SV = qs.StridedView
     ua = rng.integers(-20, 21, size=(d, d))
@@ -101,9 +103,9 @@ def bench_split(qs, torch, dev, d=64, reps=10):
     # column-major A[i,m,j,n]: storage index i + d m + d^2 j + d^3 n -> torch dims (n, j, m, i)
     A *= torch.from_numpy(np.exp2(ua)).to(dev).t()[None, :, None, :]
     B *= torch.from_numpy(np.exp2(ub)).to(dev).t()[:, None, :, None]
-    A, B = A.reshape(-1), B.reshape(-1)
-    C1 = torch.zeros(d ** 4, dtype=torch.float64, device=dev)
-    C2 = torch.zeros(d ** 4, dtype=torch.float64, device=dev)
+    A, B = A.reshape(-1).to(dtype), B.reshape(-1).to(dtype)
+    C1 = torch.zeros(d ** 4, dtype=dtype, device=dev)
+    C2 = torch.zeros(d ** 4, dtype=dtype, device=dev)
     st = colmajor((d,) * 4)
     mk = lambda Cst, prec: qs.plan_contract(SV(Cst, (d,) * 4, st), SV(A, (d,) * 4, st), (1, 5, 2, 6),
                                             SV(B, (d,) * 4, st), (5, 3, 6, 4), (1, 2, 3, 4), precision=prec)
@@ -120,7 +122,7 @@ def bench_split(qs, torch, dev, d=64, reps=10):
         _, km, kmed = time_plan(qs, torch, p, 0.0, reps, w)
         out[name + "_ms"] = round(km * 1e3, 4)
         out[name + "_ms_median"] = round(kmed * 1e3, 4)
-        out[name + "_tflops_fp64_equiv"] = round(flops / km / 1e12, 2)
+        out[name + ("_tflops_fp64_equiv" if dtype == torch.float64 else "_tflops_fp32_equiv")] = round(flops / km / 1e12, 2)
     times = time_plans_alternating(qs, torch, (plain, split), 0.0, reps, 2)
     for name, (km, _) in zip(("plain", "split"), times):
         out[name + "_ms_interleaved"] = round(km * 1e3, 4)
@@ -128,7 +130,7 @@ def bench_split(qs, torch, dev, d=64, reps=10):
     qs.execute(split, 1.0, 0.0)
     torch.cuda.synchronize()
     rep = qs.error_report(split)
-    diff = (C2 - C1).norm().item() / C1.norm().item()
+    diff = (C2.double() - C1.double()).norm().item() / C1.double().norm().item()
     out.update({"moduli": rep["moduli"], "split_vs_plain_speedup": round(out["plain_ms"] / out["split_ms"], 3),
                 "split_vs_plain_speedup_interleaved": round(out["plain_ms_interleaved"] / out["split_ms_interleaved"], 3),
                 "split_rel_fro_err_sampled_dot2": rep["rel_fro"], "split_max_rel_err_sampled": rep["max_rel"],
@@ -624,7 +626,7 @@ def main():
     ap.add_argument("--tune-cache", default=None,
                     help="autotune cache file: loaded if it exists (no tuning runs), else written after tuning")
     ap.add_argument("--only", default=None,
-                    choices=["split", "grouped_f64", "grouped_c64", "served", "bandwidth", "batched", "mixed",
+                    choices=["split", "split_f32", "grouped_f64", "grouped_c64", "served", "bandwidth", "batched", "mixed",
                              "rank4_f64", "c_anb_f64", "permuted_c64", "peaks"],
                     help="only run one of the extra workloads and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -654,6 +656,8 @@ def main():
     if args.only is not None:
         if args.only == "split":
             out = bench_split(qs, torch, dev)
+        elif args.only == "split_f32":
+            out = bench_split(qs, torch, dev, dtype=torch.float32)
         elif args.only == "bandwidth":
             out = bench_bandwidth(qs, torch, dev)
         elif args.only == "batched":
@@ -761,6 +765,7 @@ def main():
             extra[name] = bench_plain_extra(qs, torch, dev, key)
         extra["fp32"] = bench_fp32(qs, torch, dev)
         extra["split_rank4_all64_f64"] = bench_split(qs, torch, dev)
+        extra["split_rank4_all64_f32"] = bench_split(qs, torch, dev, dtype=torch.float32)
         extra["bandwidth_f64"] = bench_bandwidth(qs, torch, dev)
         extra["batched_f64"] = bench_batched(qs, torch, dev)
         extra["mixed_domain_all32"] = bench_mixed_domain(qs, torch, dev)
