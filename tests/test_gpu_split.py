@@ -75,6 +75,24 @@ def test_split_f64_ragged_and_strided(oracle):
     assert np.array_equal(got[:5], c0[:5])
 
 
+def test_split_f64_both_k_fast_ragged(oracle):
+    """Both operands K fast (one row per wave in the row-norm pass) with row
+    counts that are not multiples of the four rows a workgroup takes."""
+    case("split F64 ragged, A and B both K fast")
+    rng = np.random.default_rng(9)
+    M, K, N = 70, 131, 95
+    Av = SV(torch.from_numpy(rng.standard_normal(M * K)).to(DEV), (M, K), (K, 1))   # row-major: K fast
+    Bv = SV(torch.from_numpy(rng.standard_normal(K * N)).to(DEV), (K, N), (1, K))   # K fast
+    Cv = SV(torch.from_numpy(rng.standard_normal(M * N)).to(DEV), (M, N), (1, M))
+    c0 = storage_np(Cv.parent)
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split")
+    assert plan.path == "split" and plan.info["a_kfast"] == 1 and plan.info["b_kfast"] == 1, plan.info
+    qs.execute(plan, 0.75, 0.5)
+    torch.cuda.synchronize()
+    want = oracle_of_plan(oracle, plan, 0.75, 0.5, storage_np(plan.Astorage), storage_np(plan.Bstorage), c0)
+    assert rel_fro(storage_np(Cv.parent), want) <= 1e-13
+
+
 def test_split_f32(oracle):
     case("split F32 (target 1e-6)")
     rng = np.random.default_rng(8)
