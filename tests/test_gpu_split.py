@@ -93,6 +93,25 @@ def test_split_f64_both_k_fast_ragged(oracle):
     assert rel_fro(storage_np(Cv.parent), want) <= 1e-13
 
 
+def test_split_mixed_storage_types(oracle):
+    """Float32 A and C, Float64 B: computed in Float64 (the promoted type),
+    each operand's passes reading its own storage type; C rounded once into
+    Float32, so the result is the oracle's to within an FP32 ulp."""
+    case("split F64 compute, A and C Float32, B Float64")
+    rng = np.random.default_rng(10)
+    M, K, N = 90, 200, 70
+    Av = SV(torch.from_numpy(rng.standard_normal(M * K).astype(np.float32)).to(DEV), (M, K), (1, M))
+    Bv = SV(torch.from_numpy(rng.standard_normal(K * N)).to(DEV), (K, N), (1, K))
+    Cv = SV(torch.from_numpy(rng.standard_normal(M * N).astype(np.float32)).to(DEV), (M, N), (1, M))
+    c0 = storage_np(Cv.parent)
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), precision="split")
+    assert plan.path == "split" and plan.T == torch.float64, plan
+    qs.execute(plan, 1.0, 0.5)
+    torch.cuda.synchronize()
+    want = oracle_of_plan(oracle, plan, 1.0, 0.5, storage_np(plan.Astorage), storage_np(plan.Bstorage), c0)
+    assert rel_fro(storage_np(Cv.parent).astype(np.float64), want.astype(np.float64)) <= 1e-7
+
+
 def test_split_f32(oracle):
     case("split F32 (target 1e-6)")
     rng = np.random.default_rng(8)
