@@ -70,7 +70,7 @@ class BatchedPlan:
         _lib.lib().qs_group_destroy(self._h)
         self._h, self.tile = h, tile
 
-    def autotune(self, reps=5, tiles=(-1, 0, 1, 2, 3, 4)):
+    def autotune(self, reps=5, tiles=(-1, 0, 1, 2, 3, 4, 5)):
         """Time the batched launch with each workgroup tile (C is saved and
         restored) and keep the fastest; returns {tile: seconds}.  Each tile is
         timed as a captured graph of `reps` launches, so the per-call host

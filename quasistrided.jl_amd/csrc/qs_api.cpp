@@ -236,6 +236,7 @@ TileShape tile_shape(int tile, int ct, bool akf, bool bkf) {
     case TILE_128x64: bm = 128; bn = 64; break;
     case TILE_64x128: bm = 64; bn = 128; break;
     case TILE_32x64: bm = 32; bn = 64; break;
+    case TILE_32x32: bm = 32; bn = 32; break;
     default: break;
   }
   return TileShape{bm, bn, pick_bk_host(bm, bn, ct, akf, bkf)};

@@ -1090,6 +1090,7 @@ static hipError_t launch_gemm_typed(const GemmArgs &g, hipStream_t s) {
     case TILE_128x64: return launch_tile<R, CPLX, 128, 64>(g, s);
     case TILE_64x128: return launch_tile<R, CPLX, 64, 128>(g, s);
     case TILE_32x64: return launch_tile<R, CPLX, 32, 64>(g, s);
+    case TILE_32x32: return launch_tile<R, CPLX, 32, 32>(g, s);
     default: return launch_tile<R, CPLX, 64, 64>(g, s);
   }
 }

@@ -42,7 +42,8 @@ enum TileId : int {
   TILE_128x64 = 2,    // 4 waves, 64x32 per wave
   TILE_64x128 = 3,    // 4 waves, 32x64 per wave
   TILE_32x64 = 4,     // 4 waves, 16x32 per wave (small M)
-  TILE_COUNT = 5
+  TILE_32x32 = 5,     // 4 waves, 16x16 per wave (four workgroups per CU where the grid is small)
+  TILE_COUNT = 6
 };
 struct TileShape { int bm, bn, bk; };
 TileShape tile_shape(int tile, int compute_type, bool a_kfast, bool b_kfast);

@@ -139,7 +139,7 @@ qs_status qs_autotune(qs_plan *p, void *stream, int32_t reps, qs_tune_result *ou
   int best_tile = tile0, best_splits = splits0, count = 1;
   double best_ms = default_ms;
   const bool cplx = is_complex(p->desc.compute_type);
-  const int tiles[] = {TILE_128x128, TILE_128x64, TILE_64x128, TILE_64x64, TILE_32x64};
+  const int tiles[] = {TILE_128x128, TILE_128x64, TILE_64x128, TILE_64x64, TILE_32x64, TILE_32x32};
   const int splits_menu[] = {1, 2, 4, 8};
   for (int tile : tiles) {
     if (st != QS_OK) break;

@@ -24,7 +24,7 @@ static void fetch(long long k, int bk, long long K, const char *what, int rs, in
 }
 
 int main() {
-  const int tiles[5][2] = {{128, 128}, {64, 64}, {128, 64}, {64, 128}, {32, 64}};
+  const int tiles[6][2] = {{128, 128}, {64, 64}, {128, 64}, {64, 128}, {32, 64}, {32, 32}};
   int configs = 0;
   for (int rs : {4, 8})
     for (int pl : {1, 2})

@@ -353,7 +353,7 @@ def test_split_k(oracle, dtype):
     assert plan.info["k_splits"] > 1, plan
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
 def test_forced_split_k_ragged(oracle, tile):
     """Split-K chunks are whole stage pairs; the last split ends at a ragged K
     (the K tables' pad entries read as zero)."""
@@ -413,7 +413,7 @@ def test_span_past_2gib_takes_generic_gather():
     del st
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
 def test_every_tile_shape(oracle, tile):
     case(f"tile override {tile}")
     rng = np.random.default_rng(20 + tile)
@@ -451,7 +451,7 @@ def test_paired_loads_match_single_loads(oracle, dtype, akf):
     for m, kstride, paired in ((M, 1, True), (M - 1, 1, akf), (M, 2, False)):
         case(f"paired loads {np.dtype(dtype).name}, A {'K' if akf else 'M'}-fast, M={m}, K stride {kstride}")
         results = []
-        for tile in range(5):
+        for tile in range(6):
             Av, Bv, Cv = views(m, kstride)
             plan = run_and_check(oracle, Cv, Av, (1, 2), Bv, (2, 3), (1, 3), 1.5, 0.5,
                                  kernel=qs.MFMAKernel(tile=tile, k_splits=1), path_modes=qs.PathModes("never", "never"))
@@ -645,7 +645,7 @@ def test_backend_plan_cache_rebinds_layouts():
     assert len(pl._plan_cache) == 0
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
 def test_mixed_domain_split_k(oracle, tile):
     """Mixed-domain kernels with forced split-K (partials of both planes, the
     reduce kernel on the buffered and the 64-bit gather variants), ragged K."""

@@ -19,8 +19,8 @@ def test_k_table_pad_covers_the_gather_reach(tmp_path):
     subprocess.run(["g++", "-std=c++17", "-O1", "-o", exe, src], check=True)
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout
-    # 80 instantiations: 2 element sizes x real/complex x 5 tiles x 4 gather directions.
-    assert "configs 80" in r.stdout and "failures 0" in r.stdout, r.stdout
+    # 96 instantiations: 2 element sizes x real/complex x 6 tiles x 4 gather directions.
+    assert "configs 96" in r.stdout and "failures 0" in r.stdout, r.stdout
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs a host C++ compiler")
