@@ -34,15 +34,16 @@ struct GroupItem {
   int32_t index;        // position in the caller's plan array
 };
 
-// The tile classes a grouped launch holds: 128x64, 64x64 and 32x64 (the
-// three bodies share one register allocation, and 128x128 alone needs ~230
+// The tile classes a grouped launch holds: 128x64, 64x64, 32x64 and 32x32
+// (the bodies share one register allocation, and 128x128 alone needs ~230
 // VGPRs).  Their largest LDS footprint:
 template <typename R, bool CPLX, bool AKF, bool BKF>
 struct GroupSmem {
   static constexpr int a = TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::SMEM;
   static constexpr int c = TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::SMEM;
   static constexpr int d = TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::SMEM;
-  static constexpr int cd = c > d ? c : d;
+  static constexpr int e = TileGemm<R, CPLX, 32, 32, AKF, BKF, false>::SMEM;
+  static constexpr int cd = c > d ? (c > e ? c : e) : (d > e ? d : e);
   static constexpr int value = a > cd ? a : cd;
 };
 
@@ -90,6 +91,12 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
     switch (items[lo].cls) {
       case TILE_128x64: TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
       case TILE_32x64: TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
+      case TILE_32x32:
+        // Float64 only: other types never carry this class (grouped_class).
+        // A fourth tile body cost the complex kernels spilled registers and
+        // the Float32 kernel with both operands K fast a stack array.
+        if constexpr (!CPLX && sizeof(R) == 8) TileGemm<R, CPLX, 32, 32, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem);
+        break;
       default: TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
     }
     if (cycles) {
@@ -128,9 +135,10 @@ static hipError_t launch_grouped(int ct, int akf, int bkf, const GroupItem *item
   }
 }
 
-// The grouped kernel's tile classes: 128 x 64, 64 x 64 and 32 x 64.
+// The grouped kernel's tile classes: 128 x 64, 64 x 64, 32 x 64, and 32 x 32
+// for Float64 (other compute types take 32 x 64 in its place).
 static int grouped_class(int cls, int ct) {
-  (void)ct;
+  if (cls == TILE_32x32) return ct == QS_F64 ? TILE_32x32 : TILE_32x64;
   return (cls == TILE_128x64 || cls == TILE_32x64) ? cls : TILE_64x64;
 }
 
@@ -681,9 +689,10 @@ qs_status qs_group_autotune(qs_group_plan *g, void *stream, int32_t reps, qs_gro
     free_launch_list(ls);
     return r;
   };
-  const int cands[3] = {TILE_128x64, TILE_64x64, TILE_32x64};
-  std::vector<unsigned long long> cyc[3];
-  for (int c = 0; c < 3 && st == QS_OK; ++c) {
+  constexpr int NC = 4;
+  const int cands[NC] = {TILE_128x64, TILE_64x64, TILE_32x64, TILE_32x32};
+  std::vector<unsigned long long> cyc[NC];
+  for (int c = 0; c < NC && st == QS_OK; ++c) {
     std::vector<int> cls = fixed;
     for (auto &kv : classes)
       for (size_t i : kv.second) cls[i] = cands[c];
@@ -694,7 +703,7 @@ qs_status qs_group_autotune(qs_group_plan *g, void *stream, int32_t reps, qs_gro
   for (auto &kv : classes) {
     int best = 0;
     unsigned long long best_cost = 0;
-    for (int c = 0; c < 3 && st == QS_OK; ++c) {
+    for (int c = 0; c < NC && st == QS_OK; ++c) {
       unsigned long long cost = 0;
       for (size_t i : kv.second) cost += cyc[c][i];
       if (c == 0 || cost < best_cost) { best = c; best_cost = cost; }
@@ -735,7 +744,7 @@ qs_status qs_group_autotune(qs_group_plan *g, void *stream, int32_t reps, qs_gro
     if (before_cls[i] >= 0 && member_class(g, i) != before_cls[i]) ++changed;
   if (out) {
     out->classes = (int32_t)classes.size();
-    out->candidates = 3;
+    out->candidates = NC;
     out->members_changed = changed;
     out->before_ms = t_heur;
     out->after_ms = keep ? t_tuned : t_heur;
