@@ -54,24 +54,33 @@ struct GroupSmem {
 // Round r of the persistent loop covers tiles [r G, (r + 1) G) of the G
 // workgroups; within a round each XCD takes a contiguous run of them
 // (xcd_swizzle), so the tiles one XCD runs at once are neighbours of one
-// contraction, which share its A rows or B columns in that XCD's L2.
+// contraction, which share its A rows or B columns in that XCD's L2.  Odd
+// rounds run their slots in reverse: the items come longest K first, so the
+// workgroup that took a round's longest tile takes the next round's
+// shortest, which evens out the workgroups' totals.
 template <typename R, bool CPLX, bool AKF, bool BKF>
-__global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__restrict__ items, int n_items,
+__global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__restrict__ items,
+                                                          const int64_t *__restrict__ starts, int n_items,
                                                           int64_t total_tiles, const double *__restrict__ coef,
                                                           unsigned long long *cycles, int xcds) {
   __shared__ R smem[GroupSmem<R, CPLX, AKF, BKF>::value];
   const int64_t G = gridDim.x, slot = xcd_swizzle(blockIdx.x, G, xcds);
-  for (int64_t r0 = 0; r0 + slot < total_tiles; r0 += G) {
-    const int64_t t = r0 + slot;
+  bool odd = false;
+  for (int64_t r0 = 0; r0 < total_tiles; r0 += G, odd = !odd) {
+    const int64_t t = r0 + (odd ? G - 1 - slot : slot);
+    if (t >= total_tiles) break;   // only the last round is partial
     uint64_t t0 = 0;
     if (cycles) {
       __syncthreads();
       t0 = wall_clock64();
     }
+    // Last item with tile_start <= t, searched in the items' dense copy of
+    // tile_start (the probes of a search over the items themselves each
+    // touch another cache line).
     int lo = 0, hi = n_items - 1;
-    while (lo < hi) {   // last item with tile_start <= t
+    while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (items[mid].tile_start <= t) lo = mid; else hi = mid - 1;
+      if (starts[mid] <= t) lo = mid; else hi = mid - 1;
     }
     lo = __builtin_amdgcn_readfirstlane(lo);
     GemmArgs g = items[lo].g;
@@ -107,31 +116,39 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
 }
 
 template <typename R, bool CPLX, bool AKF, bool BKF>
-static hipError_t launch_grouped_t(const GroupItem *items, int n, int64_t total, const double *coef, int grid,
-                                   hipStream_t s, unsigned long long *cycles) {
-  hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF>), dim3(grid), dim3(256), 0, s, items, n, total, coef, cycles,
-                     device_facts().xcds);
+static hipError_t launch_grouped_t(const GroupItem *items, const int64_t *starts, int n, int64_t total,
+                                   const double *coef, int grid, hipStream_t s, unsigned long long *cycles) {
+  hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF>), dim3(grid), dim3(256), 0, s, items, starts, n, total, coef,
+                     cycles, device_facts().xcds);
   return hipGetLastError();
 }
 
 template <typename R, bool CPLX>
-static hipError_t launch_grouped_rc(int akf, int bkf, const GroupItem *items, int n, int64_t total,
-                                    const double *coef, int grid, hipStream_t s, unsigned long long *cy) {
+static hipError_t launch_grouped_rc(int akf, int bkf, const GroupItem *items, const int64_t *starts, int n,
+                                    int64_t total, const double *coef, int grid, hipStream_t s,
+                                    unsigned long long *cy) {
   switch ((akf ? 2 : 0) | (bkf ? 1 : 0)) {
-    case 0: return launch_grouped_t<R, CPLX, false, false>(items, n, total, coef, grid, s, cy);
-    case 1: return launch_grouped_t<R, CPLX, false, true>(items, n, total, coef, grid, s, cy);
-    case 2: return launch_grouped_t<R, CPLX, true, false>(items, n, total, coef, grid, s, cy);
-    default: return launch_grouped_t<R, CPLX, true, true>(items, n, total, coef, grid, s, cy);
+    case 0: return launch_grouped_t<R, CPLX, false, false>(items, starts, n, total, coef, grid, s, cy);
+    case 1: return launch_grouped_t<R, CPLX, false, true>(items, starts, n, total, coef, grid, s, cy);
+    case 2: return launch_grouped_t<R, CPLX, true, false>(items, starts, n, total, coef, grid, s, cy);
+    default: return launch_grouped_t<R, CPLX, true, true>(items, starts, n, total, coef, grid, s, cy);
   }
 }
 
-static hipError_t launch_grouped(int ct, int akf, int bkf, const GroupItem *items, int n, int64_t total,
-                                 const double *coef, int grid, hipStream_t s, unsigned long long *cy = nullptr) {
+// items: n GroupItems, each launch's table followed by the n tile_start
+// values (item_starts).
+static const int64_t *item_starts(const GroupItem *items, int n) {
+  return reinterpret_cast<const int64_t *>(items + n);
+}
+
+static hipError_t launch_grouped(int ct, int akf, int bkf, const GroupItem *items, const int64_t *starts, int n,
+                                 int64_t total, const double *coef, int grid, hipStream_t s,
+                                 unsigned long long *cy = nullptr) {
   switch (ct) {
-    case QS_F64: return launch_grouped_rc<double, false>(akf, bkf, items, n, total, coef, grid, s, cy);
-    case QS_F32: return launch_grouped_rc<float, false>(akf, bkf, items, n, total, coef, grid, s, cy);
-    case QS_C64: return launch_grouped_rc<double, true>(akf, bkf, items, n, total, coef, grid, s, cy);
-    default: return launch_grouped_rc<float, true>(akf, bkf, items, n, total, coef, grid, s, cy);
+    case QS_F64: return launch_grouped_rc<double, false>(akf, bkf, items, starts, n, total, coef, grid, s, cy);
+    case QS_F32: return launch_grouped_rc<float, false>(akf, bkf, items, starts, n, total, coef, grid, s, cy);
+    case QS_C64: return launch_grouped_rc<double, true>(akf, bkf, items, starts, n, total, coef, grid, s, cy);
+    default: return launch_grouped_rc<float, true>(akf, bkf, items, starts, n, total, coef, grid, s, cy);
   }
 }
 
@@ -295,10 +312,14 @@ static qs_status make_launches(qs_group_plan *g, const std::vector<int> &cls, vo
     l.total = start;
     l.grid = (int)std::min<int64_t>(start, 2 * (int64_t)cu_count());
     if (l.grid < 1) l.grid = 1;
-    if (dev_malloc(&l.d_items, sizeof(GroupItem) * v.size()) != hipSuccess)
+    std::vector<int64_t> st(v.size());
+    for (size_t j = 0; j < v.size(); ++j) st[j] = v[j].tile_start;
+    if (dev_malloc(&l.d_items, (sizeof(GroupItem) + sizeof(int64_t)) * v.size()) != hipSuccess)
       return fail(QS_ERR_MEMORY, "qs_group_create: device table");
     out.push_back(l);
-    if (hipMemcpy(l.d_items, v.data(), sizeof(GroupItem) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(l.d_items, v.data(), sizeof(GroupItem) * v.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy((void *)item_starts(l.d_items, l.n), st.data(), sizeof(int64_t) * v.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)
       return fail(QS_ERR_DEVICE, "qs_group_create: table upload");
   }
   return QS_OK;
@@ -373,7 +394,8 @@ static qs_status launch_all(qs_group_plan *g, hipStream_t s) {
   }
   for (const Launch &l : g->launches) {
     if (l.total == 0) continue;
-    hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, l.d_items, l.n, l.total, g->d_coef, l.grid, s);
+    hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, l.d_items, item_starts(l.d_items, l.n), l.n, l.total,
+                                  g->d_coef, l.grid, s);
     if (e != hipSuccess) return fail(QS_ERR_DEVICE, std::string("group: launch: ") + hipGetErrorString(e));
   }
   for (int i : g->standalone) {
@@ -446,14 +468,17 @@ qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vect
     if (B.h_coef) (void)pinned_free(B.h_coef);
     B.d_items = nullptr; B.h_items = nullptr; B.d_coef = nullptr; B.h_coef = nullptr; B.cap = 0;
     const size_t cap = std::max<size_t>(n, 64);
-    if (dev_malloc((void **)&B.d_items, sizeof(GroupItem) * cap) != hipSuccess ||
-        pinned_malloc((void **)&B.h_items, sizeof(GroupItem) * cap, hipHostMallocDefault) != hipSuccess ||
+    // cap GroupItems, then cap tile_start values.
+    if (dev_malloc((void **)&B.d_items, (sizeof(GroupItem) + sizeof(int64_t)) * cap) != hipSuccess ||
+        pinned_malloc((void **)&B.h_items, (sizeof(GroupItem) + sizeof(int64_t)) * cap, hipHostMallocDefault) !=
+            hipSuccess ||
         dev_malloc((void **)&B.d_coef, sizeof(double) * 4 * cap) != hipSuccess ||
         pinned_malloc((void **)&B.h_coef, sizeof(double) * 4 * cap, hipHostMallocDefault) != hipSuccess)
       return fail(QS_ERR_MEMORY, "serving: batch buffers");
     B.cap = (int)cap;
   }
   GroupItem *items = (GroupItem *)B.h_items;
+  int64_t *starts = reinterpret_cast<int64_t *>(items + B.cap);
   std::map<std::tuple<int, int, int>, std::vector<GroupItem>> classes;
   for (size_t i = 0; i < n; ++i) {
     GroupItem it;
@@ -480,16 +505,20 @@ qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vect
       start += c;
     }
     std::memcpy(items + off, v.data(), sizeof(GroupItem) * v.size());
+    for (size_t j = 0; j < v.size(); ++j) starts[off + j] = v[j].tile_start;
     ls.push_back(L{off, (int)v.size(), start, std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)});
     off += v.size();
   }
-  if (hipMemcpyAsync(B.d_items, B.h_items, sizeof(GroupItem) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+  GroupItem *d_items = (GroupItem *)B.d_items;
+  int64_t *d_starts = reinterpret_cast<int64_t *>(d_items + B.cap);
+  if (hipMemcpyAsync(d_items, items, sizeof(GroupItem) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(d_starts, starts, sizeof(int64_t) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemcpyAsync(B.d_coef, B.h_coef, sizeof(double) * 4 * n, hipMemcpyHostToDevice, s) != hipSuccess)
     return fail(QS_ERR_DEVICE, "serving: batch upload");
   for (const L &l : ls) {
     if (l.total == 0) continue;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(l.total, 2 * (int64_t)cu_count()));
-    hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, (const GroupItem *)B.d_items + l.off, l.n, l.total, B.d_coef,
+    hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, d_items + l.off, d_starts + l.off, l.n, l.total, B.d_coef,
                                   grid, s);
     if (e != hipSuccess) return fail(QS_ERR_DEVICE, std::string("serving: launch: ") + hipGetErrorString(e));
   }
@@ -665,7 +694,9 @@ qs_status qs_group_autotune(qs_group_plan *g, void *stream, int32_t reps, qs_gro
     hipError_t e = hipSuccess;
     auto all = [&](unsigned long long *cy) {
       for (const Launch &l : ls)
-        if (e == hipSuccess && l.total > 0) e = launch_grouped(l.ct, l.akf, l.bkf, l.d_items, l.n, l.total, d_coef, l.grid, s, cy);
+        if (e == hipSuccess && l.total > 0)
+          e = launch_grouped(l.ct, l.akf, l.bkf, l.d_items, item_starts(l.d_items, l.n), l.n, l.total, d_coef, l.grid,
+                             s, cy);
     };
     if (r == QS_OK) all(nullptr);
     if (r == QS_OK && cyc) {
