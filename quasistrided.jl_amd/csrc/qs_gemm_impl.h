@@ -497,15 +497,18 @@ __device__ __forceinline__ void sched_interleave() {
 // RealComplexKernel (src/microkernels/kernels.jl:125-160).  The real operand
 // keeps one LDS plane and a real gather, and each block takes two MFMAs
 // (re += ar br, im += ai br; or re += ar br, im += ar bi) instead of four.
+//
+// BKO: the K stage depth, if not the deepest that fits the LDS budget
+// (pick_bk).
 template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool MIXED, bool VEC = false, int DOM = 0,
-          int BAUX = 0>
+          int BAUX = 0, int BKO = 0>
 struct TileGemm {
   static constexpr int P = CPLX ? 2 : 1;
   static constexpr bool CA = CPLX && DOM != 2, CB = CPLX && DOM != 1;   // complex A / B
   static constexpr int PA = CA ? 2 : 1, PB = CB ? 2 : 1;
   static constexpr int NMFMA = CPLX ? (DOM ? 2 : 4) : 1;   // MFMAs per 16 x 16 block and k step
   static_assert(DOM == 0 || CPLX, "mixed domains: complex compute type");
-  static constexpr int BK = pick_bk<R, CPLX, BM, BN, AKF, BKF>();
+  static constexpr int BK = BKO ? BKO : pick_bk<R, CPLX, BM, BN, AKF, BKF>();
   static constexpr int WTM = BM / 2, WTN = BN / 2;       // 2 x 2 waves
   static constexpr int FM = WTM / 16, FN = WTN / 16;     // MFMA blocks per wave
   typedef LdsTile<R, BM, BK, AKF> LA;
@@ -914,9 +917,12 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmArgs g) {
 // (256 x 256^3 F64: 192 -> 182 us).  With one tile per coordinate the
 // coordinates keep the hardware's order, which measured faster there
 // (2048 x 64^3: 56.8 against 58.9 us).
-template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool VEC = false>
-__global__ __launch_bounds__(256, 2) void gemm_batched_kernel(const GemmArgs g) {
-  typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, false, VEC> TG;
+//
+// BKO / OCC: K stage depth (0: pick_bk) and workgroups per CU
+// (batched_shape).
+template <typename R, bool CPLX, int BM, int BN, bool AKF, bool BKF, bool VEC = false, int BKO = 0, int OCC = 2>
+__global__ __launch_bounds__(256, OCC) void gemm_batched_kernel(const GemmArgs g) {
+  typedef TileGemm<R, CPLX, BM, BN, AKF, BKF, false, VEC, 0, 0, BKO> TG;
   __shared__ R smem[TG::SMEM];
   const int64_t tiles_m = (g.M + BM - 1) / BM;
   const int64_t tiles_n = (g.N + BN - 1) / BN;
@@ -936,6 +942,23 @@ __global__ __launch_bounds__(256, 2) void gemm_batched_kernel(const GemmArgs g) 
   gz.c_base += o[2];
   TG::template run<true>(gz, tm, tn, 0, smem);
 }
+
+// K stage depth and occupancy of batched launches.  Float64 64 x 64 tiles
+// take K stages of 16 (37-41 KB of LDS instead of 73-75) and four
+// workgroups per CU (at most 128 VGPRs) instead of two: a batched launch's
+// tiles are all alike, so two per CU ran their prologues and epilogues in
+// step, with nothing to hide them.  Float64 C[i,j,b] := A[i,k,b] B[k,j,b],
+// column-major: 2048 x 64^3 58.9 -> 47.8 us; 256 x 256^3 201 -> 175 us on
+// this tile, now the best of any tile there (64 x 128 at two per CU: 183).
+// Measured and not kept: 64 x 64 with stages of 8 at four or six per CU
+// (2048 x 64^3: 49.9-50 us); 128 x 64 and 64 x 128 at three per CU with
+// stages of 16 or four per CU with stages of 8 (256 x 256^3: 174-193 us).
+template <typename R, bool CPLX, int BM, int BN>
+struct batched_shape {
+  static constexpr bool SMALL = !CPLX && sizeof(R) == 8 && BM == 64 && BN == 64;
+  static constexpr int BKO = SMALL ? 16 : 0;
+  static constexpr int OCC = SMALL ? 4 : 2;
+};
 
 // A mixed-domain GEMM (complex compute type R, DOM 1 or 2) over the four
 // gather directions.
@@ -964,31 +987,40 @@ static hipError_t launch_tile_mixed(const GemmArgs &g, hipStream_t s) {
         GemmArgs part = g;
         part.batch_off = g.batch_off + 3 * z0;
         const dim3 bgrid((unsigned)tiles, 1, (unsigned)std::min<int64_t>(65535, g.batch - z0));
+        constexpr int BKO = batched_shape<R, CPLX, BM, BN>::BKO, OCC = batched_shape<R, CPLX, BM, BN>::OCC;
         // Paired loads as in the one-contraction kernel: the batch offsets
         // move the bases, not the tables, so the pairs stay adjacent.
         if constexpr (!CPLX && BM * BN <= 64 * 128) {
           if (g.vec) {
             if (g.a_kfast && g.b_kfast)
-              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, true, true>), bgrid, dim3(256), 0, s, part);
+              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, true, true, BKO, OCC>), bgrid, dim3(256), 0, s,
+                                 part);
             else if (g.a_kfast)
-              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, false, true>), bgrid, dim3(256), 0, s, part);
+              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, false, true, BKO, OCC>), bgrid, dim3(256), 0,
+                                 s, part);
             else if (g.b_kfast)
-              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, false, true, true>), bgrid, dim3(256), 0, s, part);
+              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, false, true, true, BKO, OCC>), bgrid, dim3(256), 0,
+                                 s, part);
             else
-              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, false, false, true>), bgrid, dim3(256), 0, s, part);
+              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, false, false, true, BKO, OCC>), bgrid, dim3(256), 0,
+                                 s, part);
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
             continue;
           }
         }
         if (g.a_kfast && g.b_kfast)
-          hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, true>), bgrid, dim3(256), 0, s, part);
+          hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, true, false, BKO, OCC>), bgrid, dim3(256), 0, s,
+                             part);
         else if (g.a_kfast)
-          hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, false>), bgrid, dim3(256), 0, s, part);
+          hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, false, false, BKO, OCC>), bgrid, dim3(256), 0, s,
+                             part);
         else if (g.b_kfast)
-          hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, false, true>), bgrid, dim3(256), 0, s, part);
+          hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, false, true, false, BKO, OCC>), bgrid, dim3(256), 0, s,
+                             part);
         else
-          hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, false, false>), bgrid, dim3(256), 0, s, part);
+          hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, false, false, false, BKO, OCC>), bgrid, dim3(256), 0, s,
+                             part);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
       }
