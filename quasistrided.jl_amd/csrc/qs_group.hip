@@ -37,14 +37,14 @@ struct GroupItem {
 // The tile classes a grouped launch holds: 128x64, 64x64, 32x64 and 32x32
 // (the bodies share one register allocation, and 128x128 alone needs ~230
 // VGPRs).  Their largest LDS footprint:
-template <typename R, bool CPLX, bool AKF, bool BKF>
+template <typename R, bool CPLX, bool AKF, bool BKF, int ONLY = -1>
 struct GroupSmem {
   static constexpr int a = TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::SMEM;
   static constexpr int c = TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::SMEM;
   static constexpr int d = TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::SMEM;
   static constexpr int e = TileGemm<R, CPLX, 32, 32, AKF, BKF, false>::SMEM;
   static constexpr int cd = c > d ? (c > e ? c : e) : (d > e ? d : e);
-  static constexpr int value = a > cd ? a : cd;
+  static constexpr int value = ONLY == TILE_128x64 ? TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::SMEM : (a > cd ? a : cd);
 };
 
 // coef: per item {alpha_re, alpha_im, beta_re, beta_im}, indexed by item.index.
@@ -58,12 +58,18 @@ struct GroupSmem {
 // rounds run their slots in reverse: the items come longest K first, so the
 // workgroup that took a round's longest tile takes the next round's
 // shortest, which evens out the workgroups' totals.
-template <typename R, bool CPLX, bool AKF, bool BKF>
+//
+// ONLY == TILE_128x64: a launch whose items all run 128 x 64 tiles (the
+// padding heuristic's pick for every member of the 512-member MPS sweeps).
+// Float64 sweep 380-385k -> 401-402k contractions per second, two runs each
+// alternating on one box (the one body keeps two stages in flight);
+// ComplexF64, one stage either way, 152.1-152.3k -> 152.5-153.5k.
+template <typename R, bool CPLX, bool AKF, bool BKF, int ONLY = -1>
 __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__restrict__ items,
                                                           const int64_t *__restrict__ starts, int n_items,
                                                           int64_t total_tiles, const double *__restrict__ coef,
                                                           unsigned long long *cycles, int xcds) {
-  __shared__ R smem[GroupSmem<R, CPLX, AKF, BKF>::value];
+  __shared__ R smem[GroupSmem<R, CPLX, AKF, BKF, ONLY>::value];
   const int64_t G = gridDim.x, slot = xcd_swizzle(blockIdx.x, G, xcds);
   bool odd = false;
   for (int64_t r0 = 0; r0 < total_tiles; r0 += G, odd = !odd) {
@@ -97,6 +103,11 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
     // allocation.  (Complex types with 64 x 64 and 32 x 64 tiles only, each
     // with two stages in flight, measured slower on the 512-member ComplexF64
     // sweep: 131-132k against 147k contractions per second.)
+    if constexpr (ONLY == TILE_128x64) {
+      // Every item of the launch on 128 x 64 tiles: one tile body, which
+      // has the registers for two stages in flight where its type allows.
+      TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::template run<1>(g, tm, tn, 0, smem);
+    } else {
     switch (items[lo].cls) {
       case TILE_128x64: TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
       case TILE_32x64: TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
@@ -108,6 +119,7 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
         break;
       default: TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
     }
+    }
     if (cycles) {
       __syncthreads();
       if (threadIdx.x == 0) atomicAdd(cycles + items[lo].index, (unsigned long long)(wall_clock64() - t0));
@@ -117,7 +129,13 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
 
 template <typename R, bool CPLX, bool AKF, bool BKF>
 static hipError_t launch_grouped_t(const GroupItem *items, const int64_t *starts, int n, int64_t total,
-                                   const double *coef, int grid, hipStream_t s, unsigned long long *cycles) {
+                                   const double *coef, int grid, hipStream_t s, unsigned long long *cycles,
+                                   int only = -1) {
+  if (only == TILE_128x64) {
+    hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF, TILE_128x64>), dim3(grid), dim3(256), 0, s, items, starts, n,
+                       total, coef, cycles, device_facts().xcds);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF>), dim3(grid), dim3(256), 0, s, items, starts, n, total, coef,
                      cycles, device_facts().xcds);
   return hipGetLastError();
@@ -126,12 +144,12 @@ static hipError_t launch_grouped_t(const GroupItem *items, const int64_t *starts
 template <typename R, bool CPLX>
 static hipError_t launch_grouped_rc(int akf, int bkf, const GroupItem *items, const int64_t *starts, int n,
                                     int64_t total, const double *coef, int grid, hipStream_t s,
-                                    unsigned long long *cy) {
+                                    unsigned long long *cy, int only) {
   switch ((akf ? 2 : 0) | (bkf ? 1 : 0)) {
-    case 0: return launch_grouped_t<R, CPLX, false, false>(items, starts, n, total, coef, grid, s, cy);
-    case 1: return launch_grouped_t<R, CPLX, false, true>(items, starts, n, total, coef, grid, s, cy);
-    case 2: return launch_grouped_t<R, CPLX, true, false>(items, starts, n, total, coef, grid, s, cy);
-    default: return launch_grouped_t<R, CPLX, true, true>(items, starts, n, total, coef, grid, s, cy);
+    case 0: return launch_grouped_t<R, CPLX, false, false>(items, starts, n, total, coef, grid, s, cy, only);
+    case 1: return launch_grouped_t<R, CPLX, false, true>(items, starts, n, total, coef, grid, s, cy, only);
+    case 2: return launch_grouped_t<R, CPLX, true, false>(items, starts, n, total, coef, grid, s, cy, only);
+    default: return launch_grouped_t<R, CPLX, true, true>(items, starts, n, total, coef, grid, s, cy, only);
   }
 }
 
@@ -141,14 +159,15 @@ static const int64_t *item_starts(const GroupItem *items, int n) {
   return reinterpret_cast<const int64_t *>(items + n);
 }
 
+// only: the tile class every item of the launch runs, or -1.
 static hipError_t launch_grouped(int ct, int akf, int bkf, const GroupItem *items, const int64_t *starts, int n,
                                  int64_t total, const double *coef, int grid, hipStream_t s,
-                                 unsigned long long *cy = nullptr) {
+                                 unsigned long long *cy = nullptr, int only = -1) {
   switch (ct) {
-    case QS_F64: return launch_grouped_rc<double, false>(akf, bkf, items, starts, n, total, coef, grid, s, cy);
-    case QS_F32: return launch_grouped_rc<float, false>(akf, bkf, items, starts, n, total, coef, grid, s, cy);
-    case QS_C64: return launch_grouped_rc<double, true>(akf, bkf, items, starts, n, total, coef, grid, s, cy);
-    default: return launch_grouped_rc<float, true>(akf, bkf, items, starts, n, total, coef, grid, s, cy);
+    case QS_F64: return launch_grouped_rc<double, false>(akf, bkf, items, starts, n, total, coef, grid, s, cy, only);
+    case QS_F32: return launch_grouped_rc<float, false>(akf, bkf, items, starts, n, total, coef, grid, s, cy, only);
+    case QS_C64: return launch_grouped_rc<double, true>(akf, bkf, items, starts, n, total, coef, grid, s, cy, only);
+    default: return launch_grouped_rc<float, true>(akf, bkf, items, starts, n, total, coef, grid, s, cy, only);
   }
 }
 
@@ -181,6 +200,7 @@ int grouped_tile(int64_t M, int64_t N, int ct) {
 
 struct Launch {   // one kernel instantiation's share of the group
   int ct, akf, bkf, mixed;
+  int only = -1;    // the tile class of every item, or -1
   GroupItem *d_items = nullptr;
   int n = 0;
   int64_t total = 0;
@@ -310,6 +330,9 @@ static qs_status make_launches(qs_group_plan *g, const std::vector<int> &cls, vo
     l.mixed = 0;
     l.n = (int)v.size();
     l.total = start;
+    l.only = v[0].cls;
+    for (auto &it : v)
+      if (it.cls != l.only) l.only = -1;
     l.grid = (int)std::min<int64_t>(start, 2 * (int64_t)cu_count());
     if (l.grid < 1) l.grid = 1;
     std::vector<int64_t> st(v.size());
@@ -395,7 +418,7 @@ static qs_status launch_all(qs_group_plan *g, hipStream_t s) {
   for (const Launch &l : g->launches) {
     if (l.total == 0) continue;
     hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, l.d_items, item_starts(l.d_items, l.n), l.n, l.total,
-                                  g->d_coef, l.grid, s);
+                                  g->d_coef, l.grid, s, nullptr, l.only);
     if (e != hipSuccess) return fail(QS_ERR_DEVICE, std::string("group: launch: ") + hipGetErrorString(e));
   }
   for (int i : g->standalone) {
@@ -492,7 +515,7 @@ qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vect
     classes[std::make_tuple(it.g.compute_type, it.g.a_kfast, it.g.b_kfast)].push_back(it);
   }
   std::memcpy(B.h_coef, coef4.data(), sizeof(double) * 4 * n);
-  struct L { size_t off; int n; int64_t total; int ct, akf, bkf; };
+  struct L { size_t off; int n; int64_t total; int ct, akf, bkf, only; };
   std::vector<L> ls;
   size_t off = 0;
   for (auto &kv : classes) {
@@ -506,7 +529,11 @@ qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vect
     }
     std::memcpy(items + off, v.data(), sizeof(GroupItem) * v.size());
     for (size_t j = 0; j < v.size(); ++j) starts[off + j] = v[j].tile_start;
-    ls.push_back(L{off, (int)v.size(), start, std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)});
+    int only = v[0].cls;
+    for (auto &it : v)
+      if (it.cls != only) only = -1;
+    ls.push_back(
+        L{off, (int)v.size(), start, std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), only});
     off += v.size();
   }
   GroupItem *d_items = (GroupItem *)B.d_items;
@@ -519,7 +546,7 @@ qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vect
     if (l.total == 0) continue;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(l.total, 2 * (int64_t)cu_count()));
     hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, d_items + l.off, d_starts + l.off, l.n, l.total, B.d_coef,
-                                  grid, s);
+                                  grid, s, nullptr, l.only);
     if (e != hipSuccess) return fail(QS_ERR_DEVICE, std::string("serving: launch: ") + hipGetErrorString(e));
   }
   return QS_OK;
@@ -696,7 +723,7 @@ qs_status qs_group_autotune(qs_group_plan *g, void *stream, int32_t reps, qs_gro
       for (const Launch &l : ls)
         if (e == hipSuccess && l.total > 0)
           e = launch_grouped(l.ct, l.akf, l.bkf, l.d_items, item_starts(l.d_items, l.n), l.n, l.total, d_coef, l.grid,
-                             s, cy);
+                             s, cy, l.only);
     };
     if (r == QS_OK) all(nullptr);
     if (r == QS_OK && cyc) {

@@ -96,10 +96,11 @@ def test_group_graph_replays_equal_eager(oracle):
     grp.release()
 
 
-@pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32])
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32, np.complex64])
 def test_group_every_tile_class_matches_oracle(oracle, dtype):
     # Tile ids 0..5 forced on every member; the grouped kernel runs 128x64,
-    # 64x64, 32x64 and (Float64 only) 32x32 bodies, the others map onto them.
+    # 64x64, 32x64 and (Float64 only) 32x32 bodies, the others map onto them,
+    # and a launch whose members all take 128x64 runs the one-body kernel.
     # K order inside a tile is the same for every class, so all classes give
     # the same bits.
     case(f"group with every tile class forced {np.dtype(dtype).name}")
