@@ -81,6 +81,30 @@ def test_paired_loads_on_every_tile(dtype):
         bp.release()
 
 
+@pytest.mark.parametrize("k", [3, 16, 17, 130])
+@pytest.mark.parametrize("akf,bkf", [(False, False), (False, True), (True, False), (True, True)])
+def test_f64_64x64_shallow_stages_every_direction(k, akf, bkf):
+    # Float64 64 x 64 batched tiles run K stages of 16 at four workgroups per
+    # CU: every gather direction (A or B with K as its unit-stride axis), odd
+    # extents and an odd storage offset, K below, at and past one stage,
+    # ragged M and N.
+    case(f"batched F64 64x64 shallow stages: K={k}, A K-fast {akf}, B K-fast {bkf}")
+    rng = np.random.default_rng(1000 + k)
+    i, j, b = 67, 45, 4
+    if akf:   # A[k,i,b] storage, K unit stride
+        Av = dev_view(randn(rng, np.float64, 1 + k * i * b), (i, k, b), (k, 1, k * i), offset=1)
+    else:
+        Av = dev_view(randn(rng, np.float64, 1 + i * k * b), (i, k, b), colmajor((i, k, b)), offset=1)
+    if bkf:
+        Bv = dev_view(randn(rng, np.float64, k * j * b), (k, j, b), colmajor((k, j, b)))
+    else:     # B[j,k,b] storage, J unit stride
+        Bv = dev_view(randn(rng, np.float64, j * k * b), (k, j, b), (j, 1, j * k))
+    Cv = dev_view(randn(rng, np.float64, i * j * b), (i, j, b), colmajor((i, j, b)))
+    bp = run_batched(Cv, Av, (1, 2, 3), Bv, (2, 4, 3), (1, 4, 3), 0.75, 0.5, tile=1)
+    print("[plan]", bp.plan.info.get("paired_loads"), bp.plan, flush=True)
+    bp.release()
+
+
 def test_two_batch_labels_permuted_conj():
     case("C[b2,j,i,b1] := conj(A[b1,i,k,b2]) * B[k,b2,j,b1], ComplexF64, strided parents")
     rng = np.random.default_rng(61)
