@@ -630,7 +630,7 @@ def main():
                     help="autotune cache file: loaded if it exists (no tuning runs), else written after tuning")
     ap.add_argument("--only", default=None,
                     choices=["split", "split_f32", "grouped_f64", "grouped_c64", "served", "bandwidth", "batched", "mixed",
-                             "rank4_f64", "c_anb_f64", "permuted_c64", "peaks"],
+                             "rank4_f64", "c_anb_f64", "permuted_c64", "fp32", "peaks"],
                     help="only run one of the extra workloads and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy 5 times (PMC byte-counter calibration)")
@@ -669,6 +669,8 @@ def main():
             out = bench_mixed_domain(qs, torch, dev)
         elif args.only.startswith("grouped"):
             out = bench_grouped(qs, torch, dev, args.only.endswith("c64"))
+        elif args.only == "fp32":
+            out = bench_fp32(qs, torch, dev)
         elif args.only == "peaks":
             out = qs.measure_peaks()
         elif args.only in ("rank4_f64", "c_anb_f64", "permuted_c64"):
