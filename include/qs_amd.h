@@ -336,8 +336,10 @@ qs_status qs_autotune_clear(void);
 typedef struct qs_group_plan qs_group_plan;
 typedef struct qs_graph qs_graph;
 
-/* `tiles`: per-member workgroup tile (qs TileId 0..4, -1 = the grouped
- * heuristic), or NULL for all heuristic. */
+/* `tiles`: per-member workgroup tile (qs TileId 0..5, -1 = the grouped
+ * heuristic), or NULL for all heuristic.  The grouped kernel runs 128x64,
+ * 64x64, 32x64 and (Float64) 32x32 tiles; the other ids map onto the nearest
+ * of these. */
 qs_status qs_group_create(qs_plan *const *plans, int32_t n, const int32_t *tiles, qs_group_plan **out);
 
 /* A batched contraction as a group: n members that all run `plan`, member i
