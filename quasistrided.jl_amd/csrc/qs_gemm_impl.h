@@ -993,8 +993,8 @@ static hipError_t launch_tile_mixed(const GemmArgs &g, hipStream_t s) {
         if constexpr (!CPLX && BM * BN <= 64 * 128) {
           if (g.vec) {
             if (g.a_kfast && g.b_kfast)
-              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, true, true, BKO, OCC>), bgrid, dim3(256), 0, s,
-                                 part);
+              hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, true, true, BKO, OCC>), bgrid, dim3(256), 0,
+                                 s, part);
             else if (g.a_kfast)
               hipLaunchKernelGGL((gemm_batched_kernel<R, CPLX, BM, BN, true, false, true, BKO, OCC>), bgrid, dim3(256), 0,
                                  s, part);

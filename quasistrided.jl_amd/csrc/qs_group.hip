@@ -36,7 +36,7 @@ struct GroupItem {
 
 // The tile classes a grouped launch holds: 128x64, 64x64, 32x64 and 32x32
 // (the bodies share one register allocation, and 128x128 alone needs ~230
-// VGPRs).  Their largest LDS footprint:
+// VGPRs).  Their largest LDS footprint, or the one body's (ONLY):
 template <typename R, bool CPLX, bool AKF, bool BKF, int ONLY = -1>
 struct GroupSmem {
   static constexpr int a = TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::SMEM;
@@ -44,7 +44,7 @@ struct GroupSmem {
   static constexpr int d = TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::SMEM;
   static constexpr int e = TileGemm<R, CPLX, 32, 32, AKF, BKF, false>::SMEM;
   static constexpr int cd = c > d ? (c > e ? c : e) : (d > e ? d : e);
-  static constexpr int value = ONLY == TILE_128x64 ? TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::SMEM : (a > cd ? a : cd);
+  static constexpr int value = ONLY == TILE_128x64 ? a : (a > cd ? a : cd);
 };
 
 // coef: per item {alpha_re, alpha_im, beta_re, beta_im}, indexed by item.index.
@@ -115,7 +115,8 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
         // Float64 only: other types never carry this class (grouped_class).
         // A fourth tile body cost the complex kernels spilled registers and
         // the Float32 kernel with both operands K fast a stack array.
-        if constexpr (!CPLX && sizeof(R) == 8) TileGemm<R, CPLX, 32, 32, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem);
+        if constexpr (!CPLX && sizeof(R) == 8)
+          TileGemm<R, CPLX, 32, 32, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem);
         break;
       default: TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
     }
