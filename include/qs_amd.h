@@ -384,6 +384,14 @@ qs_status qs_group_autotune(qs_group_plan *g, void *stream, int32_t reps, qs_gro
 /* Algorithmic flops of one execute (2 per real MAC, 8 per complex MAC). */
 qs_status qs_group_flops(const qs_group_plan *g, int64_t *flops);
 
+/* How the group runs (its device tables rebuilt first if a member was
+ * rebound): `launches` grouped persistent launches (one per compute type,
+ * gather direction and domain) and `standalone` members that take a launch
+ * of their own (storage types other than the compute type's, except complex
+ * times real with every type native, or operand spans past 2 GiB).  No
+ * counterpart in the reference, which runs every contraction on its own. */
+qs_status qs_group_stats(qs_group_plan *g, int32_t *launches, int32_t *standalone);
+
 qs_status qs_group_destroy(qs_group_plan *g);
 
 /* Capture `repeats` back-to-back executes of the group (with the given

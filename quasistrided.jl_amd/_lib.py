@@ -151,6 +151,7 @@ SIGNATURES = {
                                           C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_void_p)]),
     "qs_group_execute": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]),
     "qs_group_flops": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "qs_group_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "qs_group_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(QSGroupTuneResult)]),
     "qs_group_destroy": (C.c_int, [C.c_void_p]),
     "qs_group_capture": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_void_p,

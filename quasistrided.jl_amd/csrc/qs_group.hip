@@ -37,12 +37,12 @@ struct GroupItem {
 // The tile classes a grouped launch holds: 128x64, 64x64, 32x64 and 32x32
 // (the bodies share one register allocation, and 128x128 alone needs ~230
 // VGPRs).  Their largest LDS footprint, or the one body's (ONLY):
-template <typename R, bool CPLX, bool AKF, bool BKF, int ONLY = -1>
+template <typename R, bool CPLX, bool AKF, bool BKF, int ONLY = -1, int DOM = 0>
 struct GroupSmem {
-  static constexpr int a = TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::SMEM;
-  static constexpr int c = TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::SMEM;
-  static constexpr int d = TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::SMEM;
-  static constexpr int e = TileGemm<R, CPLX, 32, 32, AKF, BKF, false>::SMEM;
+  static constexpr int a = TileGemm<R, CPLX, 128, 64, AKF, BKF, false, false, DOM>::SMEM;
+  static constexpr int c = TileGemm<R, CPLX, 64, 64, AKF, BKF, false, false, DOM>::SMEM;
+  static constexpr int d = TileGemm<R, CPLX, 32, 64, AKF, BKF, false, false, DOM>::SMEM;
+  static constexpr int e = TileGemm<R, CPLX, 32, 32, AKF, BKF, false, false, DOM>::SMEM;
   static constexpr int cd = c > d ? (c > e ? c : e) : (d > e ? d : e);
   static constexpr int value = ONLY == TILE_128x64 ? a : (a > cd ? a : cd);
 };
@@ -64,12 +64,16 @@ struct GroupSmem {
 // Float64 sweep 380-385k -> 401-402k contractions per second, two runs each
 // alternating on one box (the one body keeps two stages in flight);
 // ComplexF64, one stage either way, 152.1-152.3k -> 152.5-153.5k.
-template <typename R, bool CPLX, bool AKF, bool BKF, int ONLY = -1>
+//
+// DOM (complex compute types): the mixed-domain tiles, 1 = complex A times
+// real B, 2 = real A times complex B, for members whose storage types are
+// native (GemmArgs::dom_buf); 0 otherwise.
+template <typename R, bool CPLX, bool AKF, bool BKF, int ONLY = -1, int DOM = 0>
 __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__restrict__ items,
                                                           const int64_t *__restrict__ starts, int n_items,
                                                           int64_t total_tiles, const double *__restrict__ coef,
                                                           unsigned long long *cycles, int xcds) {
-  __shared__ R smem[GroupSmem<R, CPLX, AKF, BKF, ONLY>::value];
+  __shared__ R smem[GroupSmem<R, CPLX, AKF, BKF, ONLY, DOM>::value];
   const int64_t G = gridDim.x, slot = xcd_swizzle(blockIdx.x, G, xcds);
   bool odd = false;
   for (int64_t r0 = 0; r0 < total_tiles; r0 += G, odd = !odd) {
@@ -106,19 +110,23 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
     if constexpr (ONLY == TILE_128x64) {
       // Every item of the launch on 128 x 64 tiles: one tile body, which
       // has the registers for two stages in flight where its type allows.
-      TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::template run<1>(g, tm, tn, 0, smem);
+      TileGemm<R, CPLX, 128, 64, AKF, BKF, false, false, DOM>::template run<1>(g, tm, tn, 0, smem);
     } else {
+    typedef TileGemm<R, CPLX, 128, 64, AKF, BKF, false, false, DOM> T128x64;
+    typedef TileGemm<R, CPLX, 64, 64, AKF, BKF, false, false, DOM> T64x64;
+    typedef TileGemm<R, CPLX, 32, 64, AKF, BKF, false, false, DOM> T32x64;
+    typedef TileGemm<R, CPLX, 32, 32, AKF, BKF, false, false, DOM> T32x32;
     switch (items[lo].cls) {
-      case TILE_128x64: TileGemm<R, CPLX, 128, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
-      case TILE_32x64: TileGemm<R, CPLX, 32, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
+      case TILE_128x64: T128x64::template run<0>(g, tm, tn, 0, smem); break;
+      case TILE_32x64: T32x64::template run<0>(g, tm, tn, 0, smem); break;
       case TILE_32x32:
         // Float64 only: other types never carry this class (grouped_class).
         // A fourth tile body cost the complex kernels spilled registers and
         // the Float32 kernel with both operands K fast a stack array.
         if constexpr (!CPLX && sizeof(R) == 8)
-          TileGemm<R, CPLX, 32, 32, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem);
+          T32x32::template run<0>(g, tm, tn, 0, smem);
         break;
-      default: TileGemm<R, CPLX, 64, 64, AKF, BKF, false>::template run<0>(g, tm, tn, 0, smem); break;
+      default: T64x64::template run<0>(g, tm, tn, 0, smem); break;
     }
     }
     if (cycles) {
@@ -128,30 +136,41 @@ __global__ __launch_bounds__(256, 2) void grouped_kernel(const GroupItem *__rest
   }
 }
 
-template <typename R, bool CPLX, bool AKF, bool BKF>
+template <typename R, bool CPLX, bool AKF, bool BKF, int DOM>
 static hipError_t launch_grouped_t(const GroupItem *items, const int64_t *starts, int n, int64_t total,
                                    const double *coef, int grid, hipStream_t s, unsigned long long *cycles,
-                                   int only = -1) {
+                                   int only) {
   if (only == TILE_128x64) {
-    hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF, TILE_128x64>), dim3(grid), dim3(256), 0, s, items, starts, n,
-                       total, coef, cycles, device_facts().xcds);
+    hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF, TILE_128x64, DOM>), dim3(grid), dim3(256), 0, s, items,
+                       starts, n, total, coef, cycles, device_facts().xcds);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF>), dim3(grid), dim3(256), 0, s, items, starts, n, total, coef,
-                     cycles, device_facts().xcds);
+  hipLaunchKernelGGL((grouped_kernel<R, CPLX, AKF, BKF, -1, DOM>), dim3(grid), dim3(256), 0, s, items, starts, n,
+                     total, coef, cycles, device_facts().xcds);
   return hipGetLastError();
 }
 
+template <typename R, bool CPLX, int DOM>
+static hipError_t launch_grouped_d(int akf, int bkf, const GroupItem *items, const int64_t *starts, int n,
+                                   int64_t total, const double *coef, int grid, hipStream_t s,
+                                   unsigned long long *cy, int only) {
+  switch ((akf ? 2 : 0) | (bkf ? 1 : 0)) {
+    case 0: return launch_grouped_t<R, CPLX, false, false, DOM>(items, starts, n, total, coef, grid, s, cy, only);
+    case 1: return launch_grouped_t<R, CPLX, false, true, DOM>(items, starts, n, total, coef, grid, s, cy, only);
+    case 2: return launch_grouped_t<R, CPLX, true, false, DOM>(items, starts, n, total, coef, grid, s, cy, only);
+    default: return launch_grouped_t<R, CPLX, true, true, DOM>(items, starts, n, total, coef, grid, s, cy, only);
+  }
+}
+
 template <typename R, bool CPLX>
-static hipError_t launch_grouped_rc(int akf, int bkf, const GroupItem *items, const int64_t *starts, int n,
+static hipError_t launch_grouped_rc(int akf, int bkf, int dom, const GroupItem *items, const int64_t *starts, int n,
                                     int64_t total, const double *coef, int grid, hipStream_t s,
                                     unsigned long long *cy, int only) {
-  switch ((akf ? 2 : 0) | (bkf ? 1 : 0)) {
-    case 0: return launch_grouped_t<R, CPLX, false, false>(items, starts, n, total, coef, grid, s, cy, only);
-    case 1: return launch_grouped_t<R, CPLX, false, true>(items, starts, n, total, coef, grid, s, cy, only);
-    case 2: return launch_grouped_t<R, CPLX, true, false>(items, starts, n, total, coef, grid, s, cy, only);
-    default: return launch_grouped_t<R, CPLX, true, true>(items, starts, n, total, coef, grid, s, cy, only);
+  if constexpr (CPLX) {
+    if (dom == 1) return launch_grouped_d<R, CPLX, 1>(akf, bkf, items, starts, n, total, coef, grid, s, cy, only);
+    if (dom == 2) return launch_grouped_d<R, CPLX, 2>(akf, bkf, items, starts, n, total, coef, grid, s, cy, only);
   }
+  return launch_grouped_d<R, CPLX, 0>(akf, bkf, items, starts, n, total, coef, grid, s, cy, only);
 }
 
 // items: n GroupItems, each launch's table followed by the n tile_start
@@ -160,17 +179,20 @@ static const int64_t *item_starts(const GroupItem *items, int n) {
   return reinterpret_cast<const int64_t *>(items + n);
 }
 
-// only: the tile class every item of the launch runs, or -1.
-static hipError_t launch_grouped(int ct, int akf, int bkf, const GroupItem *items, const int64_t *starts, int n,
-                                 int64_t total, const double *coef, int grid, hipStream_t s,
-                                 unsigned long long *cy = nullptr, int only = -1) {
+// only: the tile class every item of the launch runs, or -1.  dom: the
+// items' GemmArgs::dom (0 for real compute types).
+static hipError_t launch_grouped(int ct, int akf, int bkf, int dom, const GroupItem *items, const int64_t *starts,
+                                 int n, int64_t total, const double *coef, int grid, hipStream_t s,
+                                 unsigned long long *cy, int only) {
   switch (ct) {
-    case QS_F64: return launch_grouped_rc<double, false>(akf, bkf, items, starts, n, total, coef, grid, s, cy, only);
-    case QS_F32: return launch_grouped_rc<float, false>(akf, bkf, items, starts, n, total, coef, grid, s, cy, only);
-    case QS_C64: return launch_grouped_rc<double, true>(akf, bkf, items, starts, n, total, coef, grid, s, cy, only);
-    default: return launch_grouped_rc<float, true>(akf, bkf, items, starts, n, total, coef, grid, s, cy, only);
+    case QS_F64: return launch_grouped_rc<double, false>(akf, bkf, 0, items, starts, n, total, coef, grid, s, cy, only);
+    case QS_F32: return launch_grouped_rc<float, false>(akf, bkf, 0, items, starts, n, total, coef, grid, s, cy, only);
+    case QS_C64:
+      return launch_grouped_rc<double, true>(akf, bkf, dom, items, starts, n, total, coef, grid, s, cy, only);
+    default: return launch_grouped_rc<float, true>(akf, bkf, dom, items, starts, n, total, coef, grid, s, cy, only);
   }
 }
+
 
 // The grouped kernel's tile classes: 128 x 64, 64 x 64, 32 x 64, and 32 x 32
 // for Float64 (other compute types take 32 x 64 in its place).
@@ -200,7 +222,7 @@ int grouped_tile(int64_t M, int64_t N, int ct) {
 }
 
 struct Launch {   // one kernel instantiation's share of the group
-  int ct, akf, bkf, mixed;
+  int ct, akf, bkf, dom;
   int only = -1;    // the tile class of every item, or -1
   GroupItem *d_items = nullptr;
   int n = 0;
@@ -293,7 +315,7 @@ static int member_class(const qs_group_plan *g, size_t i) {
 // launch per compute type and gather directions).  scratch != null: every
 // member writes C into `scratch` instead (the autotuner's runs).
 static qs_status make_launches(qs_group_plan *g, const std::vector<int> &cls, void *scratch, std::vector<Launch> &out) {
-  std::map<std::tuple<int, int, int>, std::vector<GroupItem>> by_launch;
+  std::map<std::tuple<int, int, int, int>, std::vector<GroupItem>> by_launch;
   for (size_t i = 0; i < g->plans.size(); ++i) {
     if (cls[i] < 0) continue;
     qs_plan *p = g->plans[i];
@@ -311,7 +333,7 @@ static qs_status make_launches(qs_group_plan *g, const std::vector<int> &cls, vo
     it.tiles_m = (p->M + ts.bm - 1) / ts.bm;
     it.tile_start = it.tiles_m * ((p->N + ts.bn - 1) / ts.bn);   // tile count, prefix-summed below
     it.index = (int32_t)i;
-    by_launch[std::make_tuple(p->desc.compute_type, p->a_kfast, p->b_kfast)].push_back(it);
+    by_launch[std::make_tuple(p->desc.compute_type, p->a_kfast, p->b_kfast, a.dom)].push_back(it);
   }
   for (auto &kv : by_launch) {
     std::vector<GroupItem> &v = kv.second;
@@ -328,7 +350,7 @@ static qs_status make_launches(qs_group_plan *g, const std::vector<int> &cls, vo
     l.ct = std::get<0>(kv.first);
     l.akf = std::get<1>(kv.first);
     l.bkf = std::get<2>(kv.first);
-    l.mixed = 0;
+    l.dom = std::get<3>(kv.first);
     l.n = (int)v.size();
     l.total = start;
     l.only = v[0].cls;
@@ -365,7 +387,7 @@ static qs_status build_tables(qs_group_plan *g) {
   for (size_t i = 0; i < g->plans.size(); ++i) {
     qs_plan *p = g->plans[i];
     GemmArgs a = make_args(p, p->desc.a, base_a(g, i), p->desc.b, base_b(g, i), p->desc.c, base_c(g, i));
-    if (a.mixed) {   // the generic gather (mixed types, spans past 2 GiB) runs standalone, on the same stream
+    if (!grouped_member(a)) {   // the generic gather runs standalone, on the same stream
       g->standalone.push_back((int)i);
       continue;
     }
@@ -418,7 +440,7 @@ static qs_status launch_all(qs_group_plan *g, hipStream_t s) {
   }
   for (const Launch &l : g->launches) {
     if (l.total == 0) continue;
-    hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, l.d_items, item_starts(l.d_items, l.n), l.n, l.total,
+    hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, l.dom, l.d_items, item_starts(l.d_items, l.n), l.n, l.total,
                                   g->d_coef, l.grid, s, nullptr, l.only);
     if (e != hipSuccess) return fail(QS_ERR_DEVICE, std::string("group: launch: ") + hipGetErrorString(e));
   }
@@ -503,7 +525,7 @@ qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vect
   }
   GroupItem *items = (GroupItem *)B.h_items;
   int64_t *starts = reinterpret_cast<int64_t *>(items + B.cap);
-  std::map<std::tuple<int, int, int>, std::vector<GroupItem>> classes;
+  std::map<std::tuple<int, int, int, int>, std::vector<GroupItem>> classes;
   for (size_t i = 0; i < n; ++i) {
     GroupItem it;
     std::memset(&it, 0, sizeof it);
@@ -513,10 +535,10 @@ qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vect
     it.tiles_m = (it.g.M + ts.bm - 1) / ts.bm;
     it.tile_start = it.tiles_m * ((it.g.N + ts.bn - 1) / ts.bn);
     it.index = (int32_t)i;
-    classes[std::make_tuple(it.g.compute_type, it.g.a_kfast, it.g.b_kfast)].push_back(it);
+    classes[std::make_tuple(it.g.compute_type, it.g.a_kfast, it.g.b_kfast, it.g.dom)].push_back(it);
   }
   std::memcpy(B.h_coef, coef4.data(), sizeof(double) * 4 * n);
-  struct L { size_t off; int n; int64_t total; int ct, akf, bkf, only; };
+  struct L { size_t off; int n; int64_t total; int ct, akf, bkf, dom, only; };
   std::vector<L> ls;
   size_t off = 0;
   for (auto &kv : classes) {
@@ -534,7 +556,8 @@ qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vect
     for (auto &it : v)
       if (it.cls != only) only = -1;
     ls.push_back(
-        L{off, (int)v.size(), start, std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), only});
+        L{off, (int)v.size(), start, std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first),
+          std::get<3>(kv.first), only});
     off += v.size();
   }
   GroupItem *d_items = (GroupItem *)B.d_items;
@@ -546,8 +569,8 @@ qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vect
   for (const L &l : ls) {
     if (l.total == 0) continue;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(l.total, 2 * (int64_t)cu_count()));
-    hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, d_items + l.off, d_starts + l.off, l.n, l.total, B.d_coef,
-                                  grid, s, nullptr, l.only);
+    hipError_t e = launch_grouped(l.ct, l.akf, l.bkf, l.dom, d_items + l.off, d_starts + l.off, l.n, l.total,
+                                  B.d_coef, grid, s, nullptr, l.only);
     if (e != hipSuccess) return fail(QS_ERR_DEVICE, std::string("serving: launch: ") + hipGetErrorString(e));
   }
   return QS_OK;
@@ -687,7 +710,7 @@ qs_status qs_group_autotune(qs_group_plan *g, void *stream, int32_t reps, qs_gro
   for (size_t i = 0; i < n; ++i) {
     const qs_plan *p = g->plans[i];
     GemmArgs a = make_args(p, p->desc.a, base_a(g, i), p->desc.b, base_b(g, i), p->desc.c, base_c(g, i));
-    if (a.mixed) continue;   // standalone members take no part
+    if (!grouped_member(a)) continue;   // standalone members take no part
     scratch_bytes = std::max(scratch_bytes, (size_t)(p->mC_hi + p->nC_hi - p->mC_lo - p->nC_lo + 1) *
                                                 (size_t)elem_bytes(p->desc.c_type));
     before_cls[i] = member_class(g, i);
@@ -723,7 +746,8 @@ qs_status qs_group_autotune(qs_group_plan *g, void *stream, int32_t reps, qs_gro
     auto all = [&](unsigned long long *cy) {
       for (const Launch &l : ls)
         if (e == hipSuccess && l.total > 0)
-          e = launch_grouped(l.ct, l.akf, l.bkf, l.d_items, item_starts(l.d_items, l.n), l.n, l.total, d_coef, l.grid,
+          e = launch_grouped(l.ct, l.akf, l.bkf, l.dom, l.d_items, item_starts(l.d_items, l.n), l.n, l.total, d_coef,
+                             l.grid,
                              s, cy, l.only);
     };
     if (r == QS_OK) all(nullptr);
@@ -814,6 +838,17 @@ qs_status qs_group_autotune(qs_group_plan *g, void *stream, int32_t reps, qs_gro
 qs_status qs_group_flops(const qs_group_plan *g, int64_t *flops) {
   if (!g || !flops) return fail(QS_ERR_ARGUMENT, "qs_group_flops: null argument");
   *flops = g->flops;
+  return QS_OK;
+}
+
+qs_status qs_group_stats(qs_group_plan *g, int32_t *launches, int32_t *standalone) {
+  if (!g || !launches || !standalone) return fail(QS_ERR_ARGUMENT, "qs_group_stats: null argument");
+  const qs_status st = refresh_tables(g);
+  if (st != QS_OK) return st;
+  int32_t n = 0;
+  for (const Launch &l : g->launches) n += l.total > 0 ? 1 : 0;
+  *launches = n;
+  *standalone = (int32_t)g->standalone.size();
   return QS_OK;
 }
 

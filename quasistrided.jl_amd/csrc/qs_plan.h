@@ -83,7 +83,11 @@ struct BatchBuffers {
   double *d_coef = nullptr, *h_coef = nullptr;
   int cap = 0;
 };
-// Launch `args` (non-mixed GEMM descriptors, K unsplit) as grouped launches
+// A member the grouped kernel can run: one storage type, or mixed domains
+// (complex times real) with every storage type native.  The generic gather
+// (other mixed types, spans past 2 GiB) runs standalone.
+inline bool grouped_member(const GemmArgs &a) { return !a.mixed || (a.dom && a.dom_buf); }
+// Launch `args` (grouped_member GEMM descriptors, K unsplit) as grouped launches
 // with tile classes `cls` and coefficients {are, aim, bre, bim} per member.
 qs_status launch_member_batch(const std::vector<GemmArgs> &args, const std::vector<int> &cls,
                               const std::vector<double> &coef4, BatchBuffers &B, hipStream_t s);

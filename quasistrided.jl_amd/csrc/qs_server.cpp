@@ -157,7 +157,7 @@ static void worker_loop(qs_server *s) {
       std::vector<Request *> solo;
       for (uint64_t id : batch) {
         Request &r = s->reqs[id];
-        if (!s->grouped || r.g.mixed) {
+        if (!s->grouped || !grouped_member(r.g)) {
           solo.push_back(&r);
           continue;
         }

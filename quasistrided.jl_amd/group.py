@@ -53,7 +53,7 @@ class Graph:
 
 class GroupPlan:
     """Many device-memory plans run as one grouped persistent launch (per
-    compute type and gather direction).  `tiles`: optional per-plan TileId
+    compute type, gather direction and domain).  `tiles`: optional per-plan TileId
     (-1 = the grouped heuristic)."""
 
     def __init__(self, plans, tiles=None):
@@ -75,6 +75,13 @@ class GroupPlan:
         n = len(self.plans)
         s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
         _check(_lib.lib().qs_group_execute(self._h, _coef(alphas, n), _coef(betas, n), C.c_void_p(s)))
+
+    def stats(self) -> dict:
+        """Grouped persistent launches per execute, and the members that run
+        on their own (qs_group_stats)."""
+        n_l, n_s = C.c_int32(), C.c_int32()
+        _check(_lib.lib().qs_group_stats(self._h, C.byref(n_l), C.byref(n_s)))
+        return {"launches": n_l.value, "standalone": n_s.value}
 
     def autotune(self, reps=3, stream=None) -> dict:
         """Time every grouped tile class per shape class of the members (as

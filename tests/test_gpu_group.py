@@ -133,6 +133,63 @@ def test_group_every_tile_class_matches_oracle(oracle, dtype):
         grp.release()
 
 
+@pytest.mark.parametrize("cdt,rdt", [(np.complex128, np.float64), (np.complex64, np.float32)])
+def test_group_mixed_domain_members_run_grouped(oracle, cdt, rdt):
+    # Complex L times real psi and real L times complex psi (C complex, every
+    # storage type native) run in grouped launches of the mixed-domain tiles,
+    # beside plain complex members: one launch per domain, none standalone.
+    # Every tile class forced in turn gives the same bits.
+    case(f"group with mixed-domain members {np.dtype(cdt).name} / {np.dtype(rdt).name}")
+    rng = np.random.default_rng(31)
+    shapes = dims(rng, 12, lo=8, hi=150)
+
+    def rnd(dt, n):
+        if np.dtype(dt).kind == "c":
+            return (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(dt)
+        return rng.standard_normal(n).astype(dt)
+
+    views, plans, starts = [], [], []
+    for i, (a, ap, b, s) in enumerate(shapes):
+        ta, tb = [(cdt, rdt), (rdt, cdt), (cdt, cdt)][i % 3]
+        L = SV(torch.from_numpy(rnd(ta, a * 5 * ap)).to(DEV), (a, 5, ap), (1, a, a * 5))
+        P = SV(torch.from_numpy(rnd(tb, ap * s * b)).to(DEV), (ap, s, b), (1, ap, ap * s))
+        Cv = SV(torch.from_numpy(rnd(cdt, a * 5 * s * b)).to(DEV), (a, 5, s, b), (1, a, a * 5, a * 5 * s))
+        views.append(Cv)
+        starts.append(storage_np(Cv.parent))
+        plans.append(qs.plan_contract(Cv, L, (1, 2, 3), P, (3, 4, 5), (1, 2, 4, 5), conjA=(i % 2 == 1)))
+    doms = {p.info["mixed_domain"] for p in plans}
+    assert doms == {0, 1, 2}, doms
+    alphas = [0.75 + 0.25j * i for i in range(len(plans))]
+    betas = [0.5 - 0.5j for _ in plans]
+    want = [oracle_of_plan(oracle, p, alphas[i], betas[i], storage_np(p.Astorage), storage_np(p.Bstorage), starts[i])
+            for i, p in enumerate(plans)]
+    first = None
+    for tiles in (None, [2] * len(plans), [1] * len(plans), [4] * len(plans)):
+        for i, v in enumerate(views):
+            v.parent.copy_(torch.from_numpy(starts[i]).to(DEV))
+        grp = qs.GroupPlan(plans, tiles=tiles)
+        st = grp.stats()
+        assert st == {"launches": 3, "standalone": 0}, st
+        grp.execute(alphas, betas)
+        torch.cuda.synchronize()
+        got = [storage_np(v.parent) for v in views]
+        for i, p in enumerate(plans):
+            assert rel_fro(got[i], want[i]) <= RTOL[p.T], (tiles, i, shapes[i], p.info["mixed_domain"])
+        if first is None:
+            first = got
+        for i in range(len(plans)):
+            assert np.array_equal(got[i], first[i]), (tiles, i)
+        grp.release()
+    # alone vs in the mix: the same bits
+    for i in (0, 1, 2):
+        views[i].parent.copy_(torch.from_numpy(starts[i]).to(DEV))
+        alone = qs.GroupPlan([plans[i]])
+        alone.execute([alphas[i]], [betas[i]])
+        torch.cuda.synchronize()
+        assert np.array_equal(storage_np(views[i].parent), first[i]), i
+        alone.release()
+
+
 def test_group_mixed_types_run_standalone(oracle):
     case("group with mixed-storage members")
     rng = np.random.default_rng(5)
@@ -146,6 +203,7 @@ def test_group_mixed_types_run_standalone(oracle):
         views.append(Cv)
         plans.append(qs.plan_contract(Cv, A, (1, 2), B, (2, 3), (1, 3)))
     grp = qs.GroupPlan(plans)
+    assert grp.stats() == {"launches": 0, "standalone": 4}
     grp.execute(1.5, 0.5)
     torch.cuda.synchronize()
     for i, p in enumerate(plans):

@@ -36,16 +36,26 @@ def permuted(rng, dtype, d):
     return Cv, A, B, (1, 2, 3, 4), (4, 5, 2, 6), (1, 3, 5, 6)
 
 
+def permuted_mixed_domain(rng, d):
+    """C[a,b,c,d] := A[a,e,b,f] * B[f,c,e,d], A and C complex, B real."""
+    A = SV(torch.from_numpy(rnd(rng, np.complex128, d ** 4)).to(DEV), (d,) * 4, colmajor((d,) * 4))
+    B = SV(torch.from_numpy(rnd(rng, np.float64, d ** 4)).to(DEV), (d,) * 4, colmajor((d,) * 4))
+    Cv = SV(torch.from_numpy(rnd(rng, np.complex128, d ** 4)).to(DEV), (d,) * 4, colmajor((d,) * 4))
+    return Cv, A, B, (1, 2, 3, 4), (4, 5, 2, 6), (1, 3, 5, 6)
+
+
 def requests(rng, n):
     out = []
     for i in range(n):
-        kind = i % 4
+        kind = i % 5
         if kind == 0:
             out.append(permuted(rng, np.float64, 8))
         elif kind == 1:
             out.append(permuted(rng, np.complex128, 12))
         elif kind == 2:
             out.append(permuted(rng, np.float32, 16))
+        elif kind == 4:
+            out.append(permuted_mixed_domain(rng, 10))
         else:
             a, k, b, nn = 24, 200, 20, 150
             A = SV(torch.from_numpy(rnd(rng, np.float64, a * k * b)).to(DEV), (a, k, b), colmajor((a, k, b)))
@@ -56,10 +66,11 @@ def requests(rng, n):
 
 
 def test_server_results_same_alone_and_in_mix(oracle):
-    case("server: 24 mixed requests, alone vs in the mix, grouped vs per-request")
+    case("server: 25 mixed requests (mixed domains among them), alone vs in the mix, grouped vs per-request")
     rng = np.random.default_rng(21)
-    reqs = requests(rng, 24)
+    reqs = requests(rng, 25)
     plans = [qs.plan_contract(Cv, A, iA, B, iB, iC) for (Cv, A, B, iA, iB, iC) in reqs]
+    assert plans[4].info["mixed_domain"] == 1
     starts = [storage_np(r[0].parent) for r in reqs]
     alphas = [1.0 + 0.1 * i for i in range(len(reqs))]
     betas = [0.5 if i % 2 else 0.0 for i in range(len(reqs))]
@@ -76,7 +87,7 @@ def test_server_results_same_alone_and_in_mix(oracle):
                               starts[i])
         assert rel_fro(mixed[i], want) <= RTOL[p.T], (i, p)
     # alone, through the same server
-    for i in (0, 1, 2, 3, 9):
+    for i in (0, 1, 2, 3, 4, 9):
         reqs[i][0].parent.copy_(torch.from_numpy(starts[i]).to(DEV))
         srv.wait(srv.submit(plans[i], alphas[i], betas[i]))
         torch.cuda.synchronize()
@@ -84,7 +95,7 @@ def test_server_results_same_alone_and_in_mix(oracle):
     srv.close()
     # per-request launches (no grouping): the same bits
     solo = qs.Server(budget_bytes=1 << 34, grouped=False)
-    for i in (0, 1, 3):
+    for i in (0, 1, 3, 4):
         reqs[i][0].parent.copy_(torch.from_numpy(starts[i]).to(DEV))
         solo.wait(solo.submit(plans[i], alphas[i], betas[i]))
         torch.cuda.synchronize()
