@@ -630,7 +630,7 @@ def main():
                     help="autotune cache file: loaded if it exists (no tuning runs), else written after tuning")
     ap.add_argument("--only", default=None,
                     choices=["split", "split_f32", "grouped_f64", "grouped_c64", "served", "bandwidth", "batched", "mixed",
-                             "rank4_f64", "c_anb_f64", "permuted_c64", "fp32", "peaks"],
+                             "rank4_f64", "c_anb_f64", "permuted_c64", "fp32", "peaks", "headline"],
                     help="only run one of the extra workloads and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy 5 times (PMC byte-counter calibration)")
@@ -671,6 +671,16 @@ def main():
             out = bench_grouped(qs, torch, dev, args.only.endswith("c64"))
         elif args.only == "fp32":
             out = bench_fp32(qs, torch, dev)
+        elif args.only == "headline":
+            # The headline plan alone: autotuned, then the timed region three
+            # times (--steps launches after --warmup), for counter runs.
+            plan, flops, _, _, beta = config2(qs, torch, np.float64, dev)
+            qs.autotune(plan, reps=10)
+            out = []
+            for _ in range(3):
+                wall, period, _, _ = time_plan_region(qs, torch, plan, beta, args.steps, args.warmup)
+                out.append({"plan": str(plan), "period_us": round(period * 1e6, 2),
+                            "wall_us": round(wall / args.steps * 1e6, 2), "frac": round(flops / period / PEAK_FP64, 4)})
         elif args.only == "peaks":
             out = qs.measure_peaks()
         elif args.only in ("rank4_f64", "c_anb_f64", "permuted_c64"):
