@@ -124,3 +124,15 @@ def test_autotune_cache_file_roundtrip(tmp_path):
     with pytest.raises(qs.ArgumentError):
         qs.autotune_load(str(tmp_path / "missing.txt"))
     qs.autotune_clear()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # One GPU is the scope: `bench.py --gpus N` with N > 1 exits with a
+    # message before it touches a device.
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2, (r.returncode, r.stderr)
+    assert "one GPU" in r.stderr and r.stdout == ""
