@@ -20,7 +20,7 @@ from .group import GroupPlan, Graph
 from .batch import BatchedPlan, batch_labels, contract_batched, plan_batched
 from .server import Server
 from .allocators import DefaultAllocator, TorchAllocator
-from .network import contraction_cost, ncon, ncon_steps, optimal_order
+from .network import contraction_cost, greedy_order, ncon, ncon_steps, optimal_order
 from . import _lib
 
 __all__ = [
@@ -31,5 +31,5 @@ __all__ = [
     "plan_contract", "planned", "release", "QuasiStridedBackend", "contraction_labels", "tensorcontract",
     "tensoradd", "tensortrace",
     "target_profile", "measure_peaks", "GroupPlan", "Graph", "Server", "BatchedPlan", "batch_labels", "contract_batched",
-    "plan_batched", "DefaultAllocator", "TorchAllocator", "ncon", "ncon_steps", "optimal_order", "contraction_cost",
+    "plan_batched", "DefaultAllocator", "TorchAllocator", "ncon", "ncon_steps", "optimal_order", "greedy_order", "contraction_cost",
 ]

@@ -13,7 +13,8 @@ a positive label appears exactly twice (contracted; twice on one tensor is a
 partial trace), a negative label once (open).  The output axes are the open
 labels in the order -1, -2, ... unless `output` says otherwise; positive
 labels are contracted in increasing order unless `order` says otherwise
-(`order="optimal"`: the least-multiplication order, `optimal_order`).
+(`order="optimal"`: the least-multiplication order, `optimal_order`;
+`order="greedy"`: `greedy_order`, for networks past the exhaustive search).
 When the next label's two tensors are chosen, every label they share is
 contracted at once; tensors left with no shared label are combined by outer
 products, first two first.
@@ -207,11 +208,55 @@ def optimal_order(network, sizes):
     return order + [l for l in positive if l not in order]
 
 
+def greedy_order(network, sizes):
+    """A contraction order (as `ncon`'s `order`) for networks of any size:
+    repeatedly contract the pair of connected nodes whose result is smallest
+    relative to its operands (result size minus the two operands' sizes,
+    ties to the cheaper step, then to the lower node numbers), listing one
+    shared label per step.  No optimality claim; `optimal_order` is exact up
+    to 16 tensors."""
+    network = [tuple(int(l) for l in ix) for ix in network]
+    ncon_steps(network)   # the same label checks as ncon
+    nodes = {i: _free_labels(ix) for i, ix in enumerate(network)}
+
+    def size(labels):
+        out = 1
+        for l in labels:
+            out *= int(sizes[l])
+        return out
+
+    order = []
+    nxt = len(nodes)
+    while True:
+        best = None
+        keys = sorted(nodes)
+        for x in range(len(keys)):
+            for y in range(x + 1, len(keys)):
+                a, b = keys[x], keys[y]
+                shared = nodes[a] & nodes[b]
+                if not shared:
+                    continue
+                result = nodes[a] ^ nodes[b]
+                score = (size(result) - size(nodes[a]) - size(nodes[b]), size(nodes[a] | nodes[b]), a, b)
+                if best is None or score < best[0]:
+                    best = (score, a, b, shared, result)
+        if best is None:
+            break
+        _, a, b, shared, result = best
+        order.append(min(shared))
+        del nodes[a], nodes[b]
+        nodes[nxt] = result
+        nxt += 1
+    positive = sorted({l for ix in network for l in ix if l > 0})
+    return order + [l for l in positive if l not in order]
+
+
 def ncon(tensors, network, conjlist=None, order=None, output=None):
     """`ncon` on the device: returns a new tensor with the open labels in
     `output` order (-1, -2, ... by default).  `conjlist[i]`: use conj of
     `tensors[i]`.  `order="optimal"` contracts in the order `optimal_order`
-    finds for these tensors' lengths."""
+    finds for these tensors' lengths (at most 16 tensors), `order="greedy"`
+    in the order `greedy_order` finds (any number)."""
     tensors = list(tensors)
     network = [tuple(ix) for ix in network]
     if len(tensors) != len(network):
@@ -229,9 +274,9 @@ def ncon(tensors, network, conjlist=None, order=None, output=None):
             if sizes.setdefault(int(l), int(n)) != int(n):
                 raise DimensionMismatch(f"ncon: label {l} has lengths {sizes[int(l)]} and {int(n)}")
     if isinstance(order, str):
-        if order != "optimal":
-            raise ArgumentError(f"ncon: order {order!r} (a label sequence or \"optimal\")")
-        order = optimal_order(network, sizes)
+        if order not in ("optimal", "greedy"):
+            raise ArgumentError(f"ncon: order {order!r} (a label sequence, \"optimal\" or \"greedy\")")
+        order = optimal_order(network, sizes) if order == "optimal" else greedy_order(network, sizes)
     traces, steps, output = ncon_steps(network, order, output)
 
     nodes = []   # (tensor, labels, conj pending)

@@ -96,6 +96,42 @@ def test_optimal_order_mps_transfer():
     assert qs.optimal_order(chain, {-1: 10, 2: 100, 1: 5, -2: 50}) == [2, 1]
     got = qs.ncon([dev(A), dev(B), dev(C)], chain, order="optimal")
     assert rel_fro(got.cpu().numpy(), A @ B @ C) <= 1e-13
+    got = qs.ncon([dev(L), dev(psi), dev(W), dev(psi)], net, order="greedy")
+    assert rel_fro(got.cpu().numpy(), want) <= 1e-13
     with pytest.raises(qs.ArgumentError, match="optimal"):
-        qs.ncon([dev(L), dev(psi), dev(W), dev(psi)], net, order="greedy")
+        qs.ncon([dev(L), dev(psi), dev(W), dev(psi)], net, order="fastest")
+    qs.plan.clear_plan_cache()
+
+
+def test_greedy_order_mps_overlap_past_the_exhaustive_search():
+    case("ncon order='greedy': <psi|psi> of a 12-site MPS (24 tensors), ComplexF64, D=16, d=2, against einsum")
+    rng = np.random.default_rng(12)
+    Lsites, D, d = 12, 16, 2
+    ket = [crandn(rng, (d, D)) / np.sqrt(D)] + [crandn(rng, (D, d, D)) / np.sqrt(D) for _ in range(Lsites - 2)] + \
+          [crandn(rng, (D, d)) / np.sqrt(D)]
+    # labels: ket bonds k_i, bra bonds b_i, physical p_i (all contracted)
+    k = lambda i: 3 * i + 1
+    b = lambda i: 3 * i + 2
+    p = lambda i: 3 * i + 3
+    net, tensors, subs = [], [], []
+    letters = "abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ"
+    sym = {}
+
+    def s_of(l):   # one einsum letter per label
+        if l not in sym:
+            sym[l] = letters[len(sym)]
+        return sym[l]
+    for i, T in enumerate(ket):
+        if i == 0:
+            kl, bl = [p(i), k(i)], [p(i), b(i)]
+        elif i == Lsites - 1:
+            kl, bl = [k(i - 1), p(i)], [b(i - 1), p(i)]
+        else:
+            kl, bl = [k(i - 1), p(i), k(i)], [b(i - 1), p(i), b(i)]
+        net += [kl, bl]
+        tensors += [T, np.conj(T)]
+        subs += ["".join(s_of(l) for l in kl), "".join(s_of(l) for l in bl)]
+    want = np.einsum(",".join(subs) + "->", *tensors, optimize="greedy")
+    got = qs.ncon([dev(t) for t in tensors], net, order="greedy")
+    assert abs(complex(got.cpu().numpy().reshape(())) - want) <= 1e-12 * abs(want)
     qs.plan.clear_plan_cache()

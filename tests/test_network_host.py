@@ -113,3 +113,53 @@ def test_optimal_order_traces_and_components():
 def test_optimal_order_rejects_bad_networks():
     with pytest.raises(qs.ArgumentError, match="contracted label 1 appears 1"):
         qs.optimal_order([[1, -1], [2, -2]], {1: 2, 2: 2, -1: 2, -2: 2})
+
+
+# ---- greedy order search (`greedy_order`), for networks past 16 tensors
+
+def test_greedy_order_matrix_chain_known_answer():
+    # (A B) leaves a 10 x 5 result from operands of 1000 and 500 entries,
+    # (B C) a 100 x 50 one: greedy takes A B first, the optimum (7500).
+    net = [[-1, 2], [2, 1], [1, -2]]
+    sizes = {-1: 10, 2: 100, 1: 5, -2: 50}
+    assert qs.greedy_order(net, sizes) == [2, 1]
+    assert qs.contraction_cost(net, sizes, qs.greedy_order(net, sizes)) == 7500
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_greedy_order_is_a_valid_order_no_cheaper_than_optimal(seed):
+    import numpy as np
+    rng = np.random.default_rng(100 + seed)
+    n = 3 + seed % 4
+    net, sizes = _random_network(rng, n, n + 1 + seed % 3, 3)
+    order = qs.greedy_order(net, sizes)
+    assert sorted(order) == sorted({l for ix in net for l in ix if l > 0})
+    assert qs.contraction_cost(net, sizes, order) >= qs.contraction_cost(net, sizes, qs.optimal_order(net, sizes))
+
+
+def test_greedy_order_traces_components_and_large_networks():
+    net = [[1, 1, -1, 2], [2, 3, -2], [3, -3], [4, -4], [4, 5], [5, -5]]
+    sizes = {1: 3, -1: 4, 2: 30, 3: 2, -2: 50, -3: 60, 4: 7, -4: 9, 5: 100, -5: 2}
+    assert sorted(qs.greedy_order(net, sizes)) == [1, 2, 3, 4, 5]
+    # an MPS overlap <psi|psi> of 12 sites: 24 tensors, past the exhaustive
+    # search; greedy runs the zipper (site by site), bond^3 * d work per step
+    L, D, d = 12, 32, 2
+    net, sizes = [], {}
+    for i in range(L):
+        left, right = 3 * i, 3 * (i + 1)         # ket bonds
+        bl, br, p = 3 * i + 1, 3 * (i + 1) + 1, 3 * i + 2   # bra bonds, physical
+        ket = ([right] if i == 0 else [left, right]) if i < L - 1 else [left]
+        bra = ([br] if i == 0 else [bl, br]) if i < L - 1 else [bl]
+        net.append(ket + [p])
+        net.append(bra + [p])
+    net = [[l + 1 for l in ix] for ix in net]
+    for ix in net:
+        for l in ix:
+            sizes[l] = d if (l - 1) % 3 == 2 else D
+    with pytest.raises(qs.ArgumentError, match="beyond the exhaustive search"):
+        qs.optimal_order(net, sizes)
+    order = qs.greedy_order(net, sizes)
+    assert sorted(order) == sorted({l for ix in net for l in ix})
+    # the zipper's cost: per site about two steps of D^3 d, far below the
+    # D^(L) of a bad order
+    assert qs.contraction_cost(net, sizes, order) <= 4 * L * D ** 3 * d
