@@ -134,29 +134,31 @@ __device__ __forceinline__ void store_elem(void *p, int t, int64_t idx, R re, R 
 // with Base's Complex expression trees (stores.jl:91-124; the library is
 // built with -ffp-contract=off so the unfused products stay unfused).  Split
 // in two so callers can issue every old-C load of a batch before the first
-// store (loads after a store to C cannot be hoisted above it).
-template <typename R, bool CPLX, bool MIXED>
+// store (loads after a store to C cannot be hoisted above it).  BC: the beta
+// case when the caller has already branched on it (-1: read it from g).
+template <typename R, bool CPLX, bool MIXED, int BC = -1>
 __device__ __forceinline__ void load_c_old(const GemmArgs &g, int64_t idx, R &o, R &oi) {
   o = 0;
   oi = 0;
-  if (g.beta_case != 0) load_elem<R, CPLX, MIXED>(g.C, g.c_type, idx, o, oi);
+  if ((BC < 0 ? g.beta_case : BC) != 0) load_elem<R, CPLX, MIXED>(g.C, g.c_type, idx, o, oi);
 }
 
-template <typename R, bool CPLX, bool MIXED, bool NT = false>
+template <typename R, bool CPLX, bool MIXED, bool NT = false, int BC = -1>
 __device__ __forceinline__ void store_c_at(const GemmArgs &g, int64_t idx, R xr, R xi, R o, R oi) {
   const R ar = (R)g.alpha_re, ai = (R)g.alpha_im;
+  const int bc = BC < 0 ? g.beta_case : BC;
   R outr, outi = 0;
   if (!CPLX) {
-    if (g.beta_case == 0) outr = ar * xr;
-    else outr = g.beta_case == 1 ? qs_fma(ar, xr, o) : qs_fma(ar, xr, (R)g.beta_re * o);
+    if (bc == 0) outr = ar * xr;
+    else outr = bc == 1 ? qs_fma(ar, xr, o) : qs_fma(ar, xr, (R)g.beta_re * o);
   } else {
-    if (g.beta_case == 0) {
+    if (bc == 0) {
       R t1 = ar * xr, t2 = ai * xi, t3 = ar * xi, t4 = ai * xr;
       outr = t1 - t2;
       outi = t3 + t4;
     } else {
       R tr = o, ti = oi;
-      if (g.beta_case == 2) {
+      if (bc == 2) {
         const R br = (R)g.beta_re, bi = (R)g.beta_im;
         R u1 = br * o, u2 = bi * oi, u3 = br * oi, u4 = bi * o;
         tr = u1 - u2;

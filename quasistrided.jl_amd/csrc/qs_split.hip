@@ -614,6 +614,17 @@ struct CrtRecArgs {
   int64_t MB, NB;
 };
 
+// Lane `l`'s value, as a wave-uniform one (l a compile-time constant).
+__device__ __forceinline__ double lane_f64(double x, int l) {
+  const int64_t b = __builtin_bit_cast(int64_t, x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, (int64_t)(((uint64_t)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ int64_t lane_i64(int64_t x, int l) {
+  const int lo = __builtin_amdgcn_readlane((int)x, l), hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
+  return (int64_t)(((uint64_t)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 // R: the compute type; MIXED: C stored in the other real type.
 template <bool MFAST, typename R, bool MIXED>
 __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const CrtConst c,
@@ -632,8 +643,27 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
     nb0 = 4 * (strip / a.MB);
   }
   const GemmArgs &g = a.g;
+  // The strip's scales and C offsets, loaded ahead of the residue words: lane
+  // `lane` finishes C's element `fixed` along the unit-stride side, and lane e
+  // (mod 16) holds the other side's element obase + e, which the store phase
+  // broadcasts.  Indices past C's extents are clamped for the loads; those
+  // elements are never stored.
+  const int64_t flim = MFAST ? g.M : g.N, olim = MFAST ? g.N : g.M;
+  const int64_t fixed = (MFAST ? mb0 : nb0) * 16 + lane;
+  const int64_t obase = (MFAST ? nb0 : mb0) * 16;
+  const RowScale *__restrict__ fsc = MFAST ? sa : sb;
+  const RowScale *__restrict__ osc = MFAST ? sb : sa;
+  const int64_t *__restrict__ foffs = MFAST ? mC : nC;
+  const int64_t *__restrict__ ooffs = MFAST ? nC : mC;
+  const int64_t fcl = fixed < flim ? fixed : flim - 1;
+  const int64_t ocl = obase + (lane & 15) < olim ? obase + (lane & 15) : olim - 1;
+  const double f_iq = fsc[fcl].iq, o_iq = osc[ocl].iq;
+  const int f_sh = fsc[fcl].sh, o_sh = osc[ocl].sh;
+  const int64_t foff = foffs[fcl], ooff = ooffs[ocl];
   const QS_GLOBAL unsigned int *u = (const QS_GLOBAL unsigned int *)ures;
   // The residue words of RB blocks are loaded before any arithmetic on them.
+  // Words past n re-read word n - 1 (their constants are zero), so the loads
+  // carry no branch.
   constexpr int RB = 2;   // four measured slower (102 against 95 us at 4096 x 4096, 13 moduli)
 #pragma unroll 1
   for (int b0 = 0; b0 < 4; b0 += RB) {
@@ -644,7 +674,7 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
     const int64_t blk = (MFAST ? mb0 + b : mb0) * a.NB + (MFAST ? nb0 : nb0 + b);
     const QS_GLOBAL unsigned int *up = u + blk * c.n * 64 + lane;
 #pragma unroll
-    for (int l = 0; l < kMaxModuli; ++l) wb[h][l] = l < c.n ? up[l * 64] : 0u;
+    for (int l = 0; l < kMaxModuli; ++l) wb[h][l] = up[(l < c.n ? l : c.n - 1) * 64];
   }
 #pragma unroll
   for (int h = 0; h < RB; ++h) {
@@ -653,7 +683,7 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
     double s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int l = 0; l < kMaxModuli; ++l) {
-      // Words past n are zero and their constants too: adding them is exact.
+      // The constants of words past n are zero: adding their products is exact.
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const double uv = (double)((w[l] >> (8 * r)) & 0xffu);
@@ -677,34 +707,36 @@ __global__ __launch_bounds__(256) void crt_rec_kernel(const CrtRecArgs a, const 
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // Lane `lane` finishes 16 elements along C's unit-stride side: all 16
-  // old-C loads are issued before the first store.
-  const int64_t fixed = MFAST ? mb0 * 16 + lane : nb0 * 16 + lane;
-  if (fixed >= (MFAST ? g.M : g.N)) return;
-  const RowScale fs = MFAST ? sa[fixed] : sb[fixed];
-  const int64_t foff = MFAST ? mC[fixed] : nC[fixed];
+  // Lane `lane` finishes 16 elements along C's unit-stride side: every
+  // old-C load is issued before the first store, and no store waits on a
+  // load of the other side's scales or offsets.
+  if (fixed >= flim) return;
+  const int nvalid = olim - obase < 16 ? (int)(olim - obase) : 16;   // wave-uniform
+  int64_t idx[16];
+  double v[16];
 #pragma unroll
-  for (int h = 0; h < 16; h += 8) {
-    int64_t idx[8];
-    bool ok[8];
-    double v[8];
-    R o[8];
+  for (int e = 0; e < 16; ++e) {
+    // Elements past C's extent take element 0's address for the old-C load
+    // (in bounds; never stored), so the loads carry no branch.
+    idx[e] = g.c_base + foff + lane_i64(ooff, e < nvalid ? e : 0);
+    v[e] = ldexp((MFAST ? tile[wave][lane][e] : tile[wave][e][lane]) * f_iq * lane_f64(o_iq, e),
+                 f_sh + __builtin_amdgcn_readlane(o_sh, e));
+  }
+  auto finish = [&](auto bcase) {
+    constexpr int BC = decltype(bcase)::value;
+    R o[16];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = h + j;
-      const int64_t other = (MFAST ? nb0 : mb0) * 16 + e;
-      ok[j] = other < (MFAST ? g.N : g.M);
-      const RowScale os = ok[j] ? (MFAST ? sb[other] : sa[other]) : RowScale{0.0, 1.0, 0, 0};
-      idx[j] = ok[j] ? g.c_base + foff + (MFAST ? nC[other] : mC[other]) : 0;
-      v[j] = ldexp((MFAST ? tile[wave][lane][e] : tile[wave][e][lane]) * fs.iq * os.iq, fs.sh + os.sh);
-      o[j] = 0;
+    for (int e = 0; e < 16; ++e) {
       R oi;
-      if (ok[j]) load_c_old<R, false, MIXED>(g, idx[j], o[j], oi);
+      load_c_old<R, false, MIXED, BC>(g, idx[e], o[e], oi);
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (ok[j]) store_c_at<R, false, MIXED, true>(g, idx[j], (R)v[j], (R)0, o[j], (R)0);
-  }
+    for (int e = 0; e < 16; ++e)
+      if (e < nvalid) store_c_at<R, false, MIXED, true, BC>(g, idx[e], (R)v[e], (R)0, o[e], (R)0);
+  };
+  if (g.beta_case == 0) finish(std::integral_constant<int, 0>());
+  else if (g.beta_case == 1) finish(std::integral_constant<int, 1>());
+  else finish(std::integral_constant<int, 2>());
 }
 
 // Error report: for sampled (i, j), the FP64 result against a Dot2
