@@ -166,6 +166,22 @@ typedef struct qs_allocator {
 qs_status qs_plan_create_with_allocator(const qs_contract_desc *desc, const qs_allocator *allocator,
                                         qs_plan **plan);
 
+/* A device memory pool (hipMemPool_t) on `device` as a qs_allocator, for
+ * callers with no allocator of their own; no counterpart in the reference,
+ * whose DefaultAllocator is Julia's GC.  Blocks are taken with
+ * hipMallocFromPoolAsync on the null stream and waited for; they go back with
+ * hipFreeAsync when a plan releases its workspace (after the library's device
+ * synchronization).  The pool keeps up to `release_threshold` bytes reserved
+ * (UINT64_MAX: everything it has grown to), so plans built and destroyed in a
+ * loop reuse its memory.  `out->ctx` owns the pool: destroy it once no plan
+ * holds a block (QS_ERR_ARGUMENT otherwise). */
+qs_status qs_pool_allocator_create(int device, uint64_t release_threshold, qs_allocator *out);
+/* Bytes the pool holds from the device and bytes handed out now, and the
+ * number of blocks plans hold. */
+qs_status qs_pool_allocator_stats(const qs_allocator *allocator, uint64_t *reserved_bytes, uint64_t *used_bytes,
+                                  int64_t *live_blocks);
+qs_status qs_pool_allocator_destroy(qs_allocator *allocator);
+
 /* release!(plan, allocator): hand the workspace back now, split-precision
  * buffer first, then the split-K partials.  The plan's info stays readable;
  * qs_execute and qs_autotune on it then fail with QS_ERR_ARGUMENT, and

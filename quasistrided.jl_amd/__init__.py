@@ -19,7 +19,7 @@ from .target import measure_peaks, target_profile
 from .group import GroupPlan, Graph
 from .batch import BatchedPlan, batch_labels, contract_batched, plan_batched
 from .server import Server
-from .allocators import DefaultAllocator, TorchAllocator
+from .allocators import DefaultAllocator, PoolAllocator, TorchAllocator
 from .network import contraction_cost, greedy_order, ncon, ncon_steps, optimal_order
 from . import _lib
 
@@ -31,5 +31,5 @@ __all__ = [
     "plan_contract", "planned", "release", "QuasiStridedBackend", "contraction_labels", "tensorcontract",
     "tensoradd", "tensortrace",
     "target_profile", "measure_peaks", "GroupPlan", "Graph", "Server", "BatchedPlan", "batch_labels", "contract_batched",
-    "plan_batched", "DefaultAllocator", "TorchAllocator", "ncon", "ncon_steps", "optimal_order", "greedy_order", "contraction_cost",
+    "plan_batched", "DefaultAllocator", "PoolAllocator", "TorchAllocator", "ncon", "ncon_steps", "optimal_order", "greedy_order", "contraction_cost",
 ]

@@ -133,6 +133,10 @@ SIGNATURES = {
     "qs_plan_get_info": (C.c_int, [C.c_void_p, C.POINTER(QSPlanInfo)]),
     "qs_plan_create_with_allocator": (C.c_int, [C.c_void_p, C.POINTER(QSAllocator), C.POINTER(C.c_void_p)]),
     "qs_plan_release": (C.c_int, [C.c_void_p]),
+    "qs_pool_allocator_create": (C.c_int, [C.c_int, C.c_uint64, C.c_void_p]),
+    "qs_pool_allocator_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                          C.POINTER(C.c_int64)]),
+    "qs_pool_allocator_destroy": (C.c_int, [C.c_void_p]),
     "qs_plan_rebind": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
                                  C.c_void_p, C.c_int64, C.c_int64]),
     "qs_plan_destroy": (C.c_int, [C.c_void_p]),

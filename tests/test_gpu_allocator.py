@@ -161,3 +161,60 @@ def test_backend_checkpoint_and_reset():
     torch.cuda.synchronize()
     assert al.marks == ["checkpoint", ("reset", 0)]
     assert rel_fro(Cd.cpu().numpy(), (A @ B).cpu().numpy()) <= 1e-13
+
+
+def test_pool_allocator_workspaces_and_reuse():
+    case("PoolAllocator: split-K partials and split-precision buffers from the library's hipMemPool, F64")
+    rng = np.random.default_rng(76)
+    pool = qs.PoolAllocator()
+    assert pool.stats()["live_blocks"] == 0 and pool.stats()["used_bytes"] == 0
+    Av, Bv, Cv = dense_views(rng, 200, 1000, 150)
+    c0 = values(Cv).copy()
+    C_lib = SV(Cv.parent.clone(), Cv.size, Cv.strides)
+    kern = qs.MFMAKernel(tile=1, k_splits=3)
+    modes = qs.PathModes("never", "never")
+    plan = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), allocator=pool, kernel=kern, path_modes=modes)
+    own = qs.plan_contract(C_lib, Av, (1, 2), Bv, (2, 3), (1, 3), kernel=kern, path_modes=modes)
+    assert plan.info["k_splits"] == 3 and plan.info["workspace_bytes"] == 3 * 200 * 150 * 8
+    st = pool.stats()
+    assert st["live_blocks"] == 1 and st["used_bytes"] >= 3 * 200 * 150 * 8
+    qs.execute(plan, 1.5, 0.5)
+    qs.execute(own, 1.5, 0.5)
+    torch.cuda.synchronize()
+    assert torch.equal(Cv.parent, C_lib.parent)   # same kernel, same split: the same bits
+    assert rel_fro(values(Cv), reference(Av, Bv, c0, 1.5, 0.5)) <= 1e-13
+    own.release()
+
+    case("PoolAllocator: split precision 512^3 against the plain FP64 result")
+    Ap, Bp, Cp = dense_views(rng, 512, 512, 512)
+    C2 = SV(torch.zeros(512 * 512, dtype=torch.float64, device=DEV), (512, 512), (1, 512))
+    split = qs.plan_contract(C2, Ap, (1, 2), Bp, (2, 3), (1, 3), precision="split", allocator=pool)
+    plain = qs.plan_contract(Cp, Ap, (1, 2), Bp, (2, 3), (1, 3))
+    assert pool.stats()["live_blocks"] >= 2
+    qs.execute(split, 1.0, 0.0)
+    qs.execute(plain, 1.0, 0.0)
+    torch.cuda.synchronize()
+    assert rel_fro(values(C2), values(Cp)) <= 1e-13
+    plain.release()
+
+    with pytest.raises(qs.ArgumentError):   # plans still hold blocks
+        pool.close()
+    split.release()
+    plan.release()
+    torch.cuda.synchronize()
+    st = pool.stats()
+    assert st["live_blocks"] == 0 and st["used_bytes"] == 0 and st["reserved_bytes"] > 0
+
+    case("PoolAllocator: a second plan reuses the reserved memory")
+    reserved = st["reserved_bytes"]
+    again = qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), allocator=pool, kernel=kern, path_modes=modes)
+    assert again.info["k_splits"] == 3 and pool.stats()["reserved_bytes"] == reserved
+    qs.execute(again, 1.5, 0.0)
+    torch.cuda.synchronize()
+    assert rel_fro(values(Cv), reference(Av, Bv, c0, 1.5, 0.0)) <= 1e-13
+    again.release()
+    pool.close()
+    with pytest.raises(ValueError):
+        pool.stats()
+    with pytest.raises(ValueError):
+        qs.plan_contract(Cv, Av, (1, 2), Bv, (2, 3), (1, 3), allocator=pool, kernel=kern, path_modes=modes)
